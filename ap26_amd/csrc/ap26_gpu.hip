@@ -1,0 +1,446 @@
+/* ap26_gpu.hip -- C ABI of libap26gpu.so (declared in include/ap26_gpu.h):
+ * context, per-K search (the plain path), tiles, SPRP batches.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/ap26_gpu.h"
+#include "ap26_host_math.h"
+#include "ap26_kernels.h"
+
+using namespace ap26;
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      if (ctx && ctx->opts.verbose)                                                 \
+        fprintf(stderr, "ap26: %s failed: %s\n", #x, hipGetErrorString(e_));       \
+      return AP26_E_DEVICE;                                                         \
+    }                                                                               \
+  } while (0)
+
+namespace {
+
+constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* 67M survivors (~39M per K) */
+constexpr int SOL_CAP = 1 << 16;
+constexpr int NCOUNT = 8; /* device counters */
+enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2 };
+
+using SieveFn = void (*)(SieveArgs);
+
+struct SieveVariant {
+  int S, NL;
+  SieveFn fn;
+  size_t lds_bytes;
+};
+
+template <int S, int NL>
+SieveVariant make_variant() {
+  return SieveVariant{S, NL, &sieve_kernel<S, NL>, (size_t)(PT.off[NL] * S + 90 * S) * 8};
+}
+
+}  // namespace
+
+struct ap26_ctx {
+  int device = 0;
+  ap26_opts opts{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  int cu_count = 0;
+  size_t lds_per_block = 0;
+  size_t global_mem = 0;
+  char name[256] = {0};
+  /* device buffers */
+  uint64_t *d_tab = nullptr;     /* 10 shifts of tables */
+  int64_t *d_n43 = nullptr;      /* [10840] */
+  int64_t *d_surv = nullptr;     /* SURV_CAP */
+  unsigned long long *d_cnt = nullptr; /* NCOUNT counters (sol count stored as u32 at C_NSOL) */
+  DevSol *d_sols = nullptr;      /* SOL_CAP */
+  SieveVariant sieve{};
+};
+
+static SieveVariant pick_variant(int S) {
+  switch (S) {
+    case 2: return make_variant<2, 45>();
+    case 1:
+    default: return make_variant<1, 45>();
+  }
+}
+
+extern "C" {
+
+const char *ap26_strerror(int code) {
+  switch (code) {
+    case AP26_OK: return "ok";
+    case AP26_E_DEVICE: return "HIP device/runtime error";
+    case AP26_E_NOMEM: return "device memory allocation failed";
+    case AP26_E_OVERFLOW: return "output capacity too small";
+    case AP26_E_RANGE: return "argument out of range";
+    case AP26_E_INTERNAL: return "internal buffer overflow";
+    case AP26_E_BUSY: return "queue full";
+    default: return "unknown error";
+  }
+}
+
+const char *ap26_version(void) { return "ap26-mi355x 0.1 (gfx950)"; }
+
+int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
+  if (!out) return AP26_E_RANGE;
+  *out = nullptr;
+  ap26_ctx *ctx = new (std::nothrow) ap26_ctx();
+  if (!ctx) return AP26_E_NOMEM;
+  if (opts) ctx->opts = *opts;
+  ctx->device = device;
+  int rc = AP26_OK;
+  do {
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) { rc = AP26_E_DEVICE; break; }
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) { rc = AP26_E_DEVICE; break; }
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->lds_per_block = prop.sharedMemPerBlock;
+    ctx->global_mem = prop.totalGlobalMem;
+    snprintf(ctx->name, sizeof(ctx->name), "%s", prop.gcnArchName);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { rc = AP26_E_DEVICE; break; }
+    for (auto &ev : ctx->ev)
+      if (hipEventCreate(&ev) != hipSuccess) { rc = AP26_E_DEVICE; break; }
+    if (rc) break;
+    if (hipMalloc(&ctx->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT) != hipSuccess ||
+        hipMalloc(&ctx->d_n43, sizeof(int64_t) * AP26_NN43) != hipSuccess ||
+        hipMalloc(&ctx->d_surv, sizeof(int64_t) * SURV_CAP) != hipSuccess ||
+        hipMalloc(&ctx->d_cnt, sizeof(unsigned long long) * NCOUNT) != hipSuccess ||
+        hipMalloc(&ctx->d_sols, sizeof(DevSol) * SOL_CAP) != hipSuccess) {
+      rc = AP26_E_NOMEM;
+      break;
+    }
+    int S = ctx->opts.shifts_per_pass > 0 ? ctx->opts.shifts_per_pass : 1;
+    if (S != 1 && S != 2) S = 1;
+    ctx->opts.shifts_per_pass = S;
+    if (ctx->opts.block <= 0) ctx->opts.block = 256;
+    if (ctx->opts.blocks_per_cu <= 0) ctx->opts.blocks_per_cu = 2;
+    ctx->sieve = pick_variant(S);
+  } while (0);
+  if (rc) {
+    ap26_close(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return AP26_OK;
+}
+
+int ap26_close(ap26_ctx *ctx) {
+  if (!ctx) return AP26_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(ctx->d_tab);
+  (void)hipFree(ctx->d_n43);
+  (void)hipFree(ctx->d_surv);
+  (void)hipFree(ctx->d_cnt);
+  (void)hipFree(ctx->d_sols);
+  for (auto &ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return AP26_OK;
+}
+
+int ap26_device_info(ap26_ctx *ctx, int32_t *cu_count, int64_t *lds_per_block, int64_t *global_mem,
+                     char *name, int32_t name_len) {
+  if (!ctx) return AP26_E_RANGE;
+  if (cu_count) *cu_count = ctx->cu_count;
+  if (lds_per_block) *lds_per_block = (int64_t)ctx->lds_per_block;
+  if (global_mem) *global_mem = (int64_t)ctx->global_mem;
+  if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", ctx->name);
+  return AP26_OK;
+}
+
+}  // extern "C"
+
+/* ---- one tile of one K --------------------------------------------------- */
+
+static double elapsed_ms(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+  return (double)ms;
+}
+
+/* Enqueue sieve of shifts [it0,it1) x n43 [j0,j1) into d_surv (counter C_NSURV).
+   Counters must have been cleared. */
+static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int j0, int j1,
+                         ap26_stats *st) {
+  const int S = ctx->sieve.S;
+  const int nslot = it1 - it0;
+  const int npass = (nslot + S - 1) / S;
+  /* tables for every shift slot of every pass */
+  dim3 tgrid((541 + 63) / 64, NPT, npass * S);
+  hipLaunchKernelGGL(tables_kernel, tgrid, dim3(64), 0, ctx->stream, kp, it0, it1, S, ctx->d_tab);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
+  const int grid = ctx->cu_count * ctx->opts.blocks_per_cu;
+  for (int p = 0; p < npass; p++) {
+    SieveArgs a;
+    a.kp = kp;
+    a.n43 = ctx->d_n43;
+    a.tab = ctx->d_tab + (size_t)p * NTAB * S;
+    a.shift0 = kp.SHIFT + 64 * (it0 + p * S);
+    a.nslot = (it1 - (it0 + p * S)) < S ? (it1 - (it0 + p * S)) : S;
+    a.blk0 = (int64_t)j0 * AP26_N53_PER_N43;
+    a.blk1 = (int64_t)j1 * AP26_N53_PER_N43;
+    a.surv = ctx->d_surv;
+    a.nsurv = ctx->d_cnt + C_NSURV;
+    a.cap = SURV_CAP;
+    hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->opts.block), 0, ctx->stream, a);
+    HIPCHK(hipGetLastError());
+    if (st) st->sieve_launches++;
+  }
+  HIPCHK(hipEventRecord(ctx->ev[2], ctx->stream));
+  if (st) st->positions += (int64_t)(j1 - j0) * AP26_N53_PER_N43 * AP26_N59_STEPS * nslot;
+  return AP26_OK;
+}
+
+static int prepare_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, KParams *kp) {
+  HostK hk;
+  host_kconsts(K, &hk);
+  kp->STEP = hk.STEP;
+  kp->n0 = hk.n0;
+  kp->S31 = hk.S[0];
+  kp->S37 = hk.S[1];
+  kp->S41 = hk.S[2];
+  kp->S43 = hk.S[3];
+  kp->S47 = hk.S[4];
+  kp->S53 = hk.S[5];
+  kp->S59 = hk.S[6];
+  kp->K = K;
+  kp->SHIFT = SHIFT;
+  std::vector<int64_t> n43(AP26_NN43);
+  host_n43(&hk, n43.data());
+  HIPCHK(hipMemcpyAsync(ctx->d_n43, n43.data(), sizeof(int64_t) * AP26_NN43, hipMemcpyHostToDevice,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream)); /* n43 is a stack-owned host buffer */
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->d_cnt, NCOUNT);
+  HIPCHK(hipGetLastError());
+  return AP26_OK;
+}
+
+static bool span_ok(int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1) {
+  if (K < 1 || K > 600000000) return false;       /* per-K constants exact below ~6.3e8 */
+  if (SHIFT < 0 || SHIFT > 1 << 20) return false; /* (jj+SHIFT)*MOD stays below 2^63 */
+  if (it0 < 0 || it1 > AP26_NSHIFT || it0 > it1) return false;
+  if (j0 < 0 || j1 > AP26_NN43 || j0 > j1) return false;
+  return true;
+}
+
+extern "C" {
+
+int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0,
+                     int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out, ap26_stats *stats) {
+  if (!ctx || !n_out || (cap > 0 && !out)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, it0, it1, j0, j1)) return AP26_E_RANGE;
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(ctx->device));
+  ap26_stats st{};
+  *n_out = 0;
+  if (it0 == it1 || j0 == j1) {
+    if (stats) *stats = st;
+    return AP26_OK;
+  }
+  KParams kp;
+  int rc = prepare_k(ctx, K, SHIFT, &kp);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, &st);
+  if (rc) return rc;
+  WalkArgs w;
+  w.surv = ctx->d_surv;
+  w.nsurv = ctx->d_cnt + C_NSURV;
+  w.cap = SURV_CAP;
+  w.STEP = kp.STEP;
+  w.K = K;
+  w.SHIFT = SHIFT;
+  w.range_id = -1;
+  w.sols = ctx->d_sols;
+  w.nsol = (unsigned int *)(ctx->d_cnt + C_NSOL);
+  w.sol_cap = SOL_CAP;
+  w.ntests = ctx->d_cnt + C_NTESTS;
+  hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+  HIPCHK(hipGetLastError());
+  st.walk_launches++;
+  HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+  unsigned long long cnt[NCOUNT];
+  HIPCHK(hipMemcpyAsync(cnt, ctx->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const unsigned long long nsurv = cnt[C_NSURV];
+  const unsigned int nsol = (unsigned int)(cnt[C_NSOL] & 0xffffffffu);
+  if (nsurv > (unsigned long long)SURV_CAP) return AP26_E_INTERNAL;
+  if (nsol > (unsigned int)SOL_CAP) return AP26_E_INTERNAL;
+  std::vector<DevSol> hs(nsol);
+  if (nsol) {
+    HIPCHK(hipMemcpy(hs.data(), ctx->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+  }
+  int ncopy = (int)nsol < cap ? (int)nsol : cap;
+  for (int i = 0; i < ncopy; i++) {
+    out[i].k = hs[i].k;
+    out[i].K = hs[i].K;
+    out[i].first = hs[i].first;
+    out[i].range_id = hs[i].range_id;
+    out[i].shift_iter = hs[i].shift_iter;
+  }
+  *n_out = (int32_t)nsol;
+  st.k_searched = 1;
+  st.survivors = (int64_t)nsurv;
+  st.prp_tests = (int64_t)cnt[C_NTESTS];
+  st.reports = nsol;
+  st.ms_tables = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+  st.ms_sieve = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+  st.ms_walk = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
+int ap26_search_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, ap26_sol *out, int32_t cap,
+                  int32_t *n_out, ap26_stats *stats) {
+  return ap26_search_span(ctx, K, SHIFT, iter_start, AP26_NSHIFT, 0, AP26_NN43, out, cap, n_out, stats);
+}
+
+int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1,
+                    int64_t *out, int64_t cap, int64_t *n_out) {
+  if (!ctx || !n_out || (cap > 0 && !out)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, it0, it1, j0, j1)) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  *n_out = 0;
+  if (it0 == it1 || j0 == j1) return AP26_OK;
+  KParams kp;
+  int rc = prepare_k(ctx, K, SHIFT, &kp);
+  if (rc) return rc;
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, nullptr);
+  if (rc) return rc;
+  unsigned long long nsurv = 0;
+  HIPCHK(hipMemcpyAsync(&nsurv, ctx->d_cnt + C_NSURV, sizeof(nsurv), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (nsurv > (unsigned long long)SURV_CAP) return AP26_E_INTERNAL;
+  *n_out = (int64_t)nsurv;
+  int64_t ncopy = (int64_t)nsurv < cap ? (int64_t)nsurv : cap;
+  if (ncopy > 0) HIPCHK(hipMemcpy(out, ctx->d_surv, sizeof(int64_t) * (size_t)ncopy, hipMemcpyDeviceToHost));
+  return (int64_t)nsurv > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
+int ap26_okok_table(ap26_ctx *ctx, int32_t K, int32_t SHIFT, uint64_t *out) {
+  if (!ctx || !out) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, 0, 1, 0, 1)) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  KParams kp;
+  HostK hk;
+  host_kconsts(K, &hk);
+  kp.STEP = hk.STEP;
+  kp.K = K;
+  kp.SHIFT = SHIFT;
+  dim3 tgrid((541 + 63) / 64, NPT, 1);
+  hipLaunchKernelGGL(tables_kernel, tgrid, dim3(64), 0, ctx->stream, kp, 0, 1, 1, ctx->d_tab);
+  HIPCHK(hipGetLastError());
+  std::vector<uint64_t> t(NTAB);
+  HIPCHK(hipMemcpyAsync(t.data(), ctx->d_tab, sizeof(uint64_t) * NTAB, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < NSP; i++) {
+    int p = PT.p[i];
+    for (int r = 0; r < p; r++) {
+      int c = (r + (p - 1) / 2) % p; /* centred index of residue r */
+      out[PT.off[i] + r] = t[PT.off[i] + c];
+    }
+  }
+  return AP26_OK;
+}
+
+int ap26_prp_batch_dev(ap26_ctx *ctx, const void *d_n, void *d_out, size_t count, double *ms) {
+  if (!ctx || (count && (!d_n || !d_out))) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (count == 0) {
+    if (ms) *ms = 0.0;
+    return AP26_OK;
+  }
+  size_t blocks = (count + 255) / 256;
+  size_t maxb = (size_t)ctx->cu_count * 64;
+  if (blocks > maxb) blocks = maxb;
+  HIPCHK(hipEventRecord(ctx->ev[4], ctx->stream));
+  hipLaunchKernelGGL(prp_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, (const uint64_t *)d_n,
+                     (uint8_t *)d_out, (uint64_t)count);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[5], ctx->stream));
+  if (ms) {
+    HIPCHK(hipEventSynchronize(ctx->ev[5]));
+    *ms = elapsed_ms(ctx->ev[4], ctx->ev[5]);
+  }
+  return AP26_OK;
+}
+
+int ap26_prp_batch(ap26_ctx *ctx, const uint64_t *n, uint8_t *out, size_t count) {
+  if (!ctx || (count && (!n || !out))) return AP26_E_RANGE;
+  if (count == 0) return AP26_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  void *dn = nullptr, *dout = nullptr;
+  if (hipMalloc(&dn, count * 8) != hipSuccess) return AP26_E_NOMEM;
+  if (hipMalloc(&dout, count) != hipSuccess) {
+    (void)hipFree(dn);
+    return AP26_E_NOMEM;
+  }
+  int rc = AP26_OK;
+  if (hipMemcpy(dn, n, count * 8, hipMemcpyHostToDevice) != hipSuccess) rc = AP26_E_DEVICE;
+  if (!rc) rc = ap26_prp_batch_dev(ctx, dn, dout, count, nullptr);
+  if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = AP26_E_DEVICE;
+  if (!rc && hipMemcpy(out, dout, count, hipMemcpyDeviceToHost) != hipSuccess) rc = AP26_E_DEVICE;
+  (void)hipFree(dn);
+  (void)hipFree(dout);
+  return rc;
+}
+
+int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out, int32_t cap,
+                      int32_t *n_out, int64_t *hash_totals, ap26_stats *stats) {
+  (void)ctx; (void)r; (void)nr; (void)out; (void)cap; (void)n_out; (void)hash_totals; (void)stats;
+  return AP26_E_RANGE; /* provided by the batched engine */
+}
+
+int ap26_dev_alloc(ap26_ctx *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) return AP26_E_NOMEM;
+  return AP26_OK;
+}
+
+int ap26_dev_free(ap26_ctx *ctx, void *dptr) {
+  if (!ctx) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipFree(dptr));
+  return AP26_OK;
+}
+
+int ap26_copy_h2d(ap26_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return AP26_OK;
+}
+
+int ap26_copy_d2h(ap26_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return AP26_OK;
+}
+
+int ap26_sync(ap26_ctx *ctx) {
+  if (!ctx) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AP26_OK;
+}
+
+}  // extern "C"

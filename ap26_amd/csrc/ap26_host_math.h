@@ -1,0 +1,56 @@
+/* ap26_host_math.h -- per-K constants and n43 starts, computed on the host
+ * (C and C++), and the K admission rule.
+ */
+#ifndef AP26_HOST_MATH_H
+#define AP26_HOST_MATH_H
+
+#include <stdint.h>
+
+#include "ap26_consts.h"
+
+typedef struct {
+  int32_t K;
+  int64_t STEP;
+  int64_t n0;
+  int64_t S[7]; /* S31, S37, S41, S43, S47, S53, S59 */
+} HostK;
+
+/* K is searched iff no prime 29..59 divides it (src/common/AP26.cpp:649-653). */
+static inline int host_will_search(int32_t K) {
+  return (K % 29) && (K % 31) && (K % 37) && (K % 41) && (K % 43) && (K % 47) && (K % 53) &&
+         (K % 59);
+}
+
+static inline int64_t host_mulmod_mod(int64_t a, int64_t k) {
+  return (int64_t)(((__int128)a * k) % AP26_MOD);
+}
+
+/* STEP = K*23#, n0 = (N0*K + N30) mod MOD, S_q = PRES_q*K mod MOD
+   (src/opencl/AP26.h:112-120).  Exact 128-bit products; the reference's
+   K = 17835q + r split yields the same values for K below ~6.3e8. */
+static inline void host_kconsts(int32_t K, HostK *hk) {
+  static const int64_t pres[7] = {AP26_PRES31, AP26_PRES37, AP26_PRES41, AP26_PRES43,
+                                  AP26_PRES47, AP26_PRES53, AP26_PRES59};
+  hk->K = K;
+  hk->STEP = (int64_t)K * AP26_PRIM23;
+  hk->n0 = (host_mulmod_mod(AP26_N0, K) + AP26_N30) % AP26_MOD;
+  for (int i = 0; i < 7; i++) hk->S[i] = host_mulmod_mod(pres[i], K);
+}
+
+/* The 10840 n43 starts in the reference's order (src/opencl/AP26.h:125-136):
+   i31 < 7, i37 < 13, i41 < 17 under the three guards, then i3 < 2, i5 < 4. */
+static inline void host_n43(const HostK *hk, int64_t *out) {
+  int c = 0;
+  for (int i31 = 0; i31 < 7; i31++)
+    for (int i37 = 0; i37 < 13; i37++) {
+      if (i37 - i31 > 10 || i31 - i37 > 4) continue;
+      for (int i41 = 0; i41 < 17; i41++) {
+        if (i41 - i31 > 14 || i41 - i37 > 14 || i31 - i41 > 4 || i37 - i41 > 10) continue;
+        const int64_t base = hk->n0 + i31 * hk->S[0] + i37 * hk->S[1] + i41 * hk->S[2];
+        for (int i3 = 0; i3 < 2; i3++)
+          for (int i5 = 0; i5 < 4; i5++) out[c++] = (base + i3 * AP26_S3 + i5 * AP26_S5) % AP26_MOD;
+      }
+    }
+}
+
+#endif

@@ -1,0 +1,293 @@
+/* ap26_kernels.h -- the AP26 kernels for gfx950: sieve tables, LDS-staged
+ * sieve, survivor walk, batched SPRP.
+ *
+ * Data layout in HBM (per K being searched):
+ *   tab   : for each pass of S shifts, NTAB*S uint64 words laid out
+ *           [prime table][centred residue][shift in pass]; the first NOK*S
+ *           words are the 83 sieve primes, the last 90*S the 7..23 filters.
+ *   n43   : int64[10840] starts of the K.
+ *   surv  : int64 survivor list (n values), appended per wave.
+ *   sols  : reports (k >= 10).
+ */
+#ifndef AP26_KERNELS_H
+#define AP26_KERNELS_H
+
+#include <utility>
+
+#include "ap26_device.h"
+
+namespace ap26 {
+
+struct KParams {
+  int64_t STEP, n0, S31, S37, S41, S43, S47, S53, S59;
+  int32_t K, SHIFT;
+};
+
+struct DevSol {
+  int32_t k, K;
+  int64_t first;
+  int32_t range_id, shift_iter;
+};
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+/* ---- table build ------------------------------------------------------- */
+/* OK_p[v] is 0 iff some term n + i*STEP (i in 0..23) is divisible by p when
+ * n == v (mod p) (src/cpu/cpusse2.cpp:31-35): v == -i*STEP.  With sp = STEP mod p
+ * != 0 that is ((p - v) * sp^-1 mod p) <= 23; with sp == 0 it is v == 0.
+ * OKOK_p[r] bit jj = OK_p[(r + (jj+SHIFT)*MOD) mod p] (src/cpu/cpusse2.cpp:37-43).
+ * For the filter primes q = 7..23 bit jj = ((r + (jj+SHIFT)*MOD) mod q != 0),
+ * the reference's per-candidate n%7..n%23 test (src/cpu/cpusse2.cpp:572-577).
+ * Grid: x = centred residue chunk (64), y = table index, z = shift slot. */
+__global__ void tables_kernel(KParams kp, int32_t it0, int32_t it1, int32_t S,
+                              uint64_t *__restrict__ tab) {
+  const int ti = blockIdx.y;
+  const int slot = blockIdx.z;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = PT.p[ti];
+  if (c >= p) return;
+  const int pass = slot / S, s = slot % S;
+  const int it = it0 + slot;
+  uint64_t *dst = tab + (size_t)pass * NTAB * S + (size_t)(PT.off[ti] + c) * S + s;
+  if (it >= it1) { *dst = 0; return; }
+  const int shift = kp.SHIFT + 64 * it;
+  const int r = (c + (p + 1) / 2) % p; /* centred entry c holds residue c-(p-1)/2 */
+  const int mp = (int)(AP26_MOD % p);
+  const bool sieve = ti < NSP;
+  int sp = 0, inv = 0;
+  if (sieve) {
+    sp = (int)(((int64_t)(kp.K % p) * (AP26_PRIM23 % p)) % p);
+    if (sp) { /* sp^(p-2) mod p */
+      int e = p - 2, b = sp;
+      inv = 1;
+      while (e) {
+        if (e & 1) inv = inv * b % p;
+        b = b * b % p;
+        e >>= 1;
+      }
+    }
+  }
+  uint64_t w = 0;
+  int js = shift % p; /* (jj + shift) mod p, advanced per bit */
+  for (int jj = 0; jj < 64; jj++) {
+    int v = (r + js * mp) % p;
+    bool ok;
+    if (!sieve) ok = v != 0;
+    else if (sp == 0) ok = v != 0;
+    else ok = ((p - v) * inv) % p > 23;
+    w |= (uint64_t)ok << jj;
+    if (++js == p) js = 0;
+  }
+  *dst = w;
+}
+
+/* ---- sieve ------------------------------------------------------------- */
+
+struct SieveArgs {
+  KParams kp;
+  const int64_t *__restrict__ n43;  /* [10840] */
+  const uint64_t *__restrict__ tab; /* this pass: NTAB*S words */
+  int32_t shift0;                   /* SHIFT of the pass's first slot */
+  int32_t nslot;                    /* live slots in this pass (<= S) */
+  int64_t blk0, blk1;               /* n53 blocks [blk0, blk1) */
+  int64_t *__restrict__ surv;
+  unsigned long long *__restrict__ nsurv;
+  int64_t cap;
+};
+
+/* n53 = (n43[j] + i43*S43 + i47*S47 + i53*S53) mod MOD for block
+   b = j*12673 + i43*667 + i47*29 + i53, i.e. the reference's nested
+   "+= S mod MOD" loops (src/cpu/cpusse2.cpp:397-407, setupn.cl:396-418). */
+__device__ __forceinline__ int64_t n53_of_block(const KParams &kp, const int64_t *n43, int64_t b) {
+  int j = (int)(b / AP26_N53_PER_N43);
+  int rem = (int)(b - (int64_t)j * AP26_N53_PER_N43);
+  int i43 = rem / 667;
+  rem -= i43 * 667;
+  int i47 = rem / 29;
+  int i53 = rem - i47 * 29;
+  int64_t v = n43[j] + i43 * kp.S43 + i47 * kp.S47 + i53 * kp.S53; /* < 70*MOD < 2^56 */
+  int64_t q = (int64_t)((double)v * (1.0 / (double)AP26_MOD));
+  v -= q * AP26_MOD;
+  if (v < 0) v += AP26_MOD;
+  if (v >= AP26_MOD) v -= AP26_MOD;
+  return v;
+}
+
+/* Early-exit points: after the first 15 primes (61..131), then every 4. */
+__host__ __device__ constexpr bool check_after(int i) { return i == 14 || (i > 14 && ((i - 14) % 4 == 0)); }
+
+/* One thread walks the 35 positions n59 = n53 + i59*S59 (mod MOD) of an n53
+ * block and, for each, ANDs the 64-bit masks of S shifts over the 83 sieve
+ * primes (LDS for the first NL, the rest from L2), with early exit; survivors
+ * of the 7..23 filter are appended per wave with one atomic.  The reference's
+ * sieve (src/opencl/kernels/sieve.cl:28-152, src/cpu/cpusse2.cpp:410-614). */
+template <int S, int NL>
+__global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
+  constexpr int LMAIN = PT.off[NL] * S;
+  constexpr int LFILT = 90 * S;
+  __shared__ uint64_t lds[LMAIN + LFILT];
+  for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) lds[i] = a.tab[i];
+  for (int i = threadIdx.x; i < LFILT; i += blockDim.x) lds[LMAIN + i] = a.tab[NOK * S + i];
+  __syncthreads();
+
+  const uint64_t *__restrict__ gtab = a.tab;
+  const int lane = __lane_id();
+  const double S59d = (double)a.kp.S59;
+  const double MODd = (double)AP26_MOD;
+  const double WRAP = MODd + TWO52;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+
+  constexpr int SCALE = 8 * S; /* bytes per centred entry */
+  const char *ldsb = (const char *)lds;
+  const char *gtb = (const char *)gtab;
+
+  for (int64_t b = a.blk0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < a.blk1; b += stride) {
+    double xd = (double)n53_of_block(a.kp, a.n43, b) + TWO52;
+    for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
+      const double x = xd - TWO52;
+      const uint32_t xs = lo32(xd) * (uint32_t)SCALE;
+      uint64_t m[S];
+#pragma unroll
+      for (int s = 0; s < S; s++) m[s] = ~(uint64_t)0;
+      bool alive = true;
+      static_for<0, NSP>([&](auto IC) {
+        constexpr int I = decltype(IC)::value;
+        constexpr int P = PT.p[I];
+        if (alive) {
+          const int bo = centred_byteoff<P, SCALE>(x, xs);
+          constexpr int C0 = SCALE * (PT.off[I] + (P - 1) / 2); /* entry of residue 0 */
+          if constexpr (I < NL) {
+#pragma unroll
+            for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(ldsb + C0 + bo + 8 * s);
+          } else {
+#pragma unroll
+            for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(gtb + (int64_t)(C0 + bo + 8 * s));
+          }
+          if constexpr (check_after(I) || I == NSP - 1) {
+            uint64_t any = 0;
+#pragma unroll
+            for (int s = 0; s < S; s++) any |= m[s];
+            alive = any != 0;
+          }
+        }
+      });
+      if (alive) {
+        static_for<NSP, NPT>([&](auto IC) {
+          constexpr int I = decltype(IC)::value;
+          constexpr int P = PT.p[I];
+          const int bo = centred_byteoff<P, SCALE>(x, xs);
+          constexpr int C0 = 8 * LMAIN + SCALE * (PT.off[I] - NOK + (P - 1) / 2);
+#pragma unroll
+          for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(ldsb + C0 + bo + 8 * s);
+        });
+        const int64_t n59 = (int64_t)(__double_as_longlong(xd) & ((INT64_C(1) << 52) - 1));
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          uint64_t w = m[s];
+          const int64_t nbase = n59 + (int64_t)(a.shift0 + 64 * s) * AP26_MOD;
+          while (true) {
+            const bool has = w != 0;
+            const uint64_t bal = __ballot(has);
+            if (bal == 0) break;
+            const int cnt = __popcll(bal);
+            const int leader = __ffsll((unsigned long long)bal) - 1;
+            const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(a.nsurv, (unsigned long long)cnt);
+            base = __shfl(base, leader);
+            if (has) {
+              const int bit = __ffsll((unsigned long long)w) - 1;
+              w &= w - 1;
+              const unsigned long long idx = base + (unsigned long long)pre;
+              if (idx < (unsigned long long)a.cap) a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
+            }
+          }
+        }
+      }
+      xd += S59d;
+      if (xd >= WRAP) xd -= MODd;
+    }
+  }
+}
+
+/* ---- walk -------------------------------------------------------------- */
+/* For each survivor n: count base-2 SPRPs forward from n+5*STEP; only if the
+ * count reaches 10, extend backward from n+4*STEP while the term is positive;
+ * report (k, m+STEP) if k >= 10 (src/cpu/cpusse2.cpp:570-590,
+ * src/opencl/kernels/checkn.cl:160-202). */
+struct WalkArgs {
+  const int64_t *__restrict__ surv;
+  const unsigned long long *__restrict__ nsurv;
+  int64_t cap;
+  int64_t STEP;
+  int32_t K, SHIFT, range_id;
+  DevSol *__restrict__ sols;
+  unsigned int *__restrict__ nsol;
+  int32_t sol_cap;
+  unsigned long long *__restrict__ ntests;
+};
+
+__global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
+  unsigned long long n = *a.nsurv;
+  if (n > (unsigned long long)a.cap) n = (unsigned long long)a.cap;
+  unsigned long long tests = 0;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    const int64_t c = a.surv[i];
+    int k = 0;
+    int64_t m = c + 5 * a.STEP;
+    while (true) {
+      tests++;
+      if (!sprp2((uint64_t)m)) break;
+      k++;
+      m += a.STEP;
+    }
+    if (k >= 10) {
+      m = c + 4 * a.STEP;
+      while (m > 0) {
+        tests++;
+        if (!sprp2((uint64_t)m)) break;
+        k++;
+        m -= a.STEP;
+      }
+      if (k >= 10) {
+        unsigned int idx = atomicAdd(a.nsol, 1u);
+        if (idx < (unsigned int)a.sol_cap) {
+          DevSol s;
+          s.k = k;
+          s.K = a.K;
+          s.first = m + a.STEP;
+          s.range_id = a.range_id;
+          s.shift_iter = (int32_t)((c / AP26_MOD - a.SHIFT) / 64);
+          a.sols[idx] = s;
+        }
+      }
+    }
+  }
+  if (tests) atomicAdd(a.ntests, tests);
+}
+
+/* ---- batched SPRP ------------------------------------------------------ */
+__global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restrict__ n, uint8_t *__restrict__ out,
+                                                   uint64_t count) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = sprp2(n[i]) ? 1 : 0;
+}
+
+/* Clear counters with a kernel (not a memset node), so the sequence can be
+   captured into a graph. */
+__global__ void zero_kernel(unsigned long long *p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+}  // namespace ap26
+
+#endif

@@ -1,0 +1,136 @@
+/* ap26_gpu.h -- C ABI of the MI355X (gfx950) AP26 search library
+ * (libap26gpu.so).  Plain C types only: the host (AP26 CLI, Python ctypes
+ * mirror) never sees HIP or torch types.
+ *
+ * The library owns device memory and streams inside an ap26_ctx; callers own
+ * every host buffer they pass.  Calls on one ctx must not overlap (no internal
+ * lock); several ctx may coexist on one device.  Every entry point returns 0 or
+ * a negative AP26_E* code (ap26_strerror); none exits the process -- the host
+ * decides, as the reference's exit(EXIT_FAILURE) paths did.
+ */
+#ifndef AP26_GPU_H
+#define AP26_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AP26_OK 0
+#define AP26_E_DEVICE (-1)    /* HIP runtime / device error */
+#define AP26_E_NOMEM (-2)     /* device allocation failed */
+#define AP26_E_OVERFLOW (-3)  /* caller's output capacity too small (count still set) */
+#define AP26_E_RANGE (-4)     /* argument out of the supported range */
+#define AP26_E_INTERNAL (-5)  /* internal buffer overflow (should not happen) */
+#define AP26_E_BUSY (-6)      /* serving queue full */
+
+/* One reported progression (k terms, difference K*23#, first term).
+   range_id names the request/range it came from (batch/serving), else -1. */
+typedef struct {
+  int32_t k;
+  int32_t K;
+  int64_t first;
+  int32_t range_id;
+  int32_t shift_iter; /* which of the 10 shifts found it (0..9) */
+} ap26_sol;
+
+/* One work unit: the reference's `AP26 KMIN KMAX SHIFT`, restricted to K in
+   [kmin, kmax] that pass will_search, starting at shift iteration iter_start
+   for K == kmin (a resumed unit), 0 for the others. */
+typedef struct {
+  int32_t kmin;
+  int32_t kmax;
+  int32_t shift;
+  int32_t iter_start;
+} ap26_range;
+
+/* Tuning knobs (0 = library default / autotuner cache). */
+typedef struct {
+  int32_t shifts_per_pass;  /* 1, 2, 5 or 10 shifts sieved per residue pass */
+  int32_t block;            /* threads per workgroup of the sieve kernel */
+  int32_t blocks_per_cu;    /* resident workgroups per CU (grid = CUs x this) */
+  int32_t verbose;
+} ap26_opts;
+
+/* Per-call measurements (device time from HIP events on the ctx stream). */
+typedef struct {
+  int64_t k_searched;       /* K that passed will_search */
+  int64_t k_skipped;
+  int64_t positions;        /* n59 positions x shifts evaluated */
+  int64_t survivors;        /* sieve survivors (candidates walked) */
+  int64_t prp_tests;        /* base-2 SPRP tests run by the walk */
+  int64_t reports;          /* (k >= 10) reports */
+  double ms_tables;         /* table-build kernel time */
+  double ms_sieve;          /* sieve kernel time */
+  double ms_walk;           /* walk kernel time */
+  double ms_total;          /* wall time of the call */
+  int32_t sieve_launches;
+  int32_t walk_launches;
+} ap26_stats;
+
+typedef struct ap26_ctx ap26_ctx;
+
+const char *ap26_strerror(int code);
+const char *ap26_version(void);
+
+/* Open a context on HIP device `device` (opts may be NULL). */
+int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out);
+int ap26_close(ap26_ctx *ctx);
+/* Device properties the tuning uses. */
+int ap26_device_info(ap26_ctx *ctx, int32_t *cu_count, int64_t *lds_per_block,
+                     int64_t *global_mem, char *name, int32_t name_len);
+
+/* Search one K over its 10 shifts SHIFT+64*iter, iter in [iter_start, 10):
+   the per-K seam of the reference, replacing
+     SearchAP26(int K, int startSHIFT, int s_local, int compute)  src/opencl/AP26.h:78
+     Search_avx2(int K, int startSHIFT, int ITER, int K_COUNT, int K_DONE)  src/cpu/cpuavx2.h:15
+   Raw reports are written to out (cap entries, *n_out = total found); the
+   caller applies ReportSolution (hash, 9-base check, SOL records).  Blocking. */
+int ap26_search_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, ap26_sol *out,
+                  int32_t cap, int32_t *n_out, ap26_stats *stats);
+
+/* As ap26_search_k over shift iterations [it0, it1) and n43 starts [j0, j1)
+   (0 <= j0 < j1 <= 10840) -- a tile of one K's search, used by tests and the
+   autotuner. */
+int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1,
+                     int32_t j0, int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out,
+                     ap26_stats *stats);
+
+/* Sieve survivors only (the candidates the walk would test) of that tile,
+   in no particular order.  *n_out = total; AP26_E_OVERFLOW if > cap. */
+int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1,
+                    int32_t j0, int32_t j1, int64_t *out, int64_t cap, int64_t *n_out);
+
+/* Packed OKOK tables of one (K, SHIFT) as the sieve uses them, re-centred to
+   the reference's layout: out[offset(p) + r] = OKOK_p[r] (23693 words). */
+int ap26_okok_table(ap26_ctx *ctx, int32_t K, int32_t SHIFT, uint64_t *out);
+
+/* Base-2 strong probable-prime test of count odd 64-bit values (host buffers):
+   out[i] = 1 iff n[i] is a base-2 SPRP (textbook; 0 for n < 3 or even n).
+   Replaces PrimeQ(int64_t) (src/common/PrimeQ_x86.h:160), without its 2^62 cap. */
+int ap26_prp_batch(ap26_ctx *ctx, const uint64_t *n, uint8_t *out, size_t count);
+
+/* Same on device-resident buffers (from ap26_dev_alloc), on the ctx stream;
+   *ms = kernel time from HIP events around the launch (NULL to skip). */
+int ap26_prp_batch_dev(ap26_ctx *ctx, const void *d_n, void *d_out, size_t count, double *ms);
+
+/* Many work units in one persistent launch sequence (work-stealing K queue,
+   captured hipGraph).  out gets every report with range_id = index into r;
+   hash_totals[i] = exact sum over range i of (first % 1000 + k) for the raw
+   reports (the host applies the reference's cap and Non-Solution rules). */
+int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out,
+                      int32_t cap, int32_t *n_out, int64_t *hash_totals, ap26_stats *stats);
+
+/* Device buffer helpers (so callers need no HIP/torch to stage inputs). */
+int ap26_dev_alloc(ap26_ctx *ctx, size_t bytes, void **dptr);
+int ap26_dev_free(ap26_ctx *ctx, void *dptr);
+int ap26_copy_h2d(ap26_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ap26_copy_d2h(ap26_ctx *ctx, void *dst, const void *src, size_t bytes);
+int ap26_sync(ap26_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
