@@ -36,14 +36,18 @@ enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2 };
 using SieveFn = void (*)(SieveArgs);
 
 struct SieveVariant {
-  int S, NL;
+  int S, NL, block;
   SieveFn fn;
-  size_t lds_bytes;
 };
 
 template <int S, int NL>
 SieveVariant make_variant() {
-  return SieveVariant{S, NL, &sieve_kernel<S, NL>, (size_t)(PT.off[NL] * S + 90 * S) * 8};
+  return SieveVariant{S, NL, 256, &sieve_kernel<S, NL>};
+}
+
+template <int NL, int WAVES>
+SieveVariant make_qvariant() {
+  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES>};
 }
 
 }  // namespace
@@ -66,11 +70,12 @@ struct ap26_ctx {
   SieveVariant sieve{};
 };
 
-static SieveVariant pick_variant(int S) {
-  switch (S) {
+/* variant ids: 0 = queued sieve (default), 1 = simple S=1, 2 = simple S=2 */
+static SieveVariant pick_variant(int id) {
+  switch (id) {
+    case 1: return make_variant<1, 45>();
     case 2: return make_variant<2, 45>();
-    case 1:
-    default: return make_variant<1, 45>();
+    default: return make_qvariant<57, 8>();
   }
 }
 
@@ -120,12 +125,9 @@ int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
       rc = AP26_E_NOMEM;
       break;
     }
-    int S = ctx->opts.shifts_per_pass > 0 ? ctx->opts.shifts_per_pass : 1;
-    if (S != 1 && S != 2) S = 1;
-    ctx->opts.shifts_per_pass = S;
-    if (ctx->opts.block <= 0) ctx->opts.block = 256;
-    if (ctx->opts.blocks_per_cu <= 0) ctx->opts.blocks_per_cu = 2;
-    ctx->sieve = pick_variant(S);
+    ctx->sieve = pick_variant(ctx->opts.sieve_variant);
+    ctx->opts.block = ctx->sieve.block;
+    if (ctx->opts.blocks_per_cu <= 0) ctx->opts.blocks_per_cu = ctx->sieve.block > 256 ? 1 : 2;
   } while (0);
   if (rc) {
     ap26_close(ctx);
@@ -196,7 +198,7 @@ static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int
     a.surv = ctx->d_surv;
     a.nsurv = ctx->d_cnt + C_NSURV;
     a.cap = SURV_CAP;
-    hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->opts.block), 0, ctx->stream, a);
+    hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
   }

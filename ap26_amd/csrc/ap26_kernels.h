@@ -216,6 +216,191 @@ __global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
   }
 }
 
+/* ---- compacted sieve --------------------------------------------------- */
+/* The mask of a position dies after ~18 primes on average but the tail is
+ * long, so lanes that evaluate positions side by side diverge badly.  Here the
+ * 83 primes are cut into levels; every lane runs level 0 for its own position,
+ * and the positions whose mask is still non-zero are appended (ballot + mbcnt)
+ * to a per-wave LDS queue.  Whenever a queue holds 64 entries the wave pops a
+ * full wave of them and runs the next level on dense lanes, appending its
+ * survivors to the next queue; the last level also applies the 7..23 filter
+ * and writes the survivors out.  Levels have no internal exits.  Tables of the
+ * first NL primes live in LDS, the rest are read through L1/L2. */
+
+/* level boundaries (prime indices): [0,15) 61..131, [15,29) 137..199,
+   [29,45) 211..307, [45,83) 311..541 */
+constexpr int QLEVELS = 4;
+constexpr int QB[QLEVELS + 1] = {0, 15, 29, 45, 83};
+
+struct QEntry {
+  double xd;  /* n59 + 2^52 */
+  uint64_t m; /* mask so far */
+};
+
+template <int NL, int B, int E>
+__device__ __forceinline__ uint64_t apply_primes(uint64_t m, double xd, const char *ldsb, const char *gtb) {
+  const double x = xd - TWO52;
+  const uint32_t xs = lo32(xd) * 8u;
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    constexpr int P = PT.p[I];
+    const int bo = centred_byteoff<P, 8>(x, xs);
+    constexpr int C0 = 8 * (PT.off[I] + (P - 1) / 2);
+    if constexpr (I < NL)
+      m &= *(const uint64_t *)(ldsb + C0 + bo);
+    else
+      m &= *(const uint64_t *)(gtb + (int64_t)(C0 + bo));
+  });
+  return m;
+}
+
+template <int NL>
+__device__ __forceinline__ uint64_t apply_filters(uint64_t m, double xd, const char *ldsb) {
+  const double x = xd - TWO52;
+  const uint32_t xs = lo32(xd) * 8u;
+  constexpr int LMAIN = PT.off[NL];
+  static_for<NSP, NPT>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    constexpr int P = PT.p[I];
+    const int bo = centred_byteoff<P, 8>(x, xs);
+    constexpr int C0 = 8 * (LMAIN + PT.off[I] - NOK + (P - 1) / 2);
+    m &= *(const uint64_t *)(ldsb + C0 + bo);
+  });
+  return m;
+}
+
+/* Append the survivors n = n59 + (bit + shift0)*MOD of this lane's mask; one
+   atomic per wave per round. */
+__device__ __forceinline__ void emit_survivors(uint64_t w, double xd, const SieveArgs &a, int lane) {
+  const int64_t n59 = (int64_t)(__double_as_longlong(xd) & ((INT64_C(1) << 52) - 1));
+  const int64_t nbase = n59 + (int64_t)a.shift0 * AP26_MOD;
+  while (true) {
+    const bool has = w != 0;
+    const uint64_t bal = __ballot(has);
+    if (bal == 0) break;
+    const int cnt = __popcll(bal);
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.nsurv, (unsigned long long)cnt);
+    base = __shfl(base, leader);
+    if (has) {
+      const int bit = __ffsll((unsigned long long)w) - 1;
+      w &= w - 1;
+      const unsigned long long idx = base + (unsigned long long)pre;
+      if (idx < (unsigned long long)a.cap) a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
+    }
+  }
+}
+
+template <int NL, int WAVES>
+struct QSieve {
+  static constexpr int QCAP = 128;
+  QEntry (*q)[QCAP];  /* this wave's QLEVELS-1 queues */
+  int qn[QLEVELS - 1];
+  const char *ldsb, *gtb;
+  int lane;
+  const SieveArgs *a;
+
+  __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
+    const uint64_t bal = __ballot(pred);
+    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
+    qn[qi] += __popcll(bal);
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  /* run level L+1 on entries popped from queue L (lanes with pred) */
+  template <int L>
+  __device__ __forceinline__ void run_level(const QEntry &e, bool pred) {
+    uint64_t m = e.m;
+    if (pred) m = apply_primes<NL, QB[L + 1], QB[L + 2]>(m, e.xd, ldsb, gtb);
+    if constexpr (L + 2 == QLEVELS) {
+      if (pred && m) m = apply_filters<NL>(m, e.xd, ldsb);
+      emit_survivors(pred ? m : 0, e.xd, *a, lane);
+    } else {
+      push(L + 1, e.xd, m, pred && m != 0);
+      drain<L + 1>();
+    }
+  }
+
+  /* pop full waves from queue L while it holds >= 64 entries */
+  template <int L>
+  __device__ __forceinline__ void drain() {
+    while (qn[L] >= 64) {
+      qn[L] -= 64;
+      const QEntry e = q[L][qn[L] + lane];
+      __builtin_amdgcn_wave_barrier();
+      run_level<L>(e, true);
+    }
+  }
+
+  /* flush the partial queues, lowest level first */
+  template <int L>
+  __device__ __forceinline__ void flush() {
+    if constexpr (L < QLEVELS - 1) {
+      drain<L>();
+      const int n = qn[L];
+      if (n > 0) {
+        const bool pred = lane < n;
+        QEntry e{TWO52, 0};
+        if (pred) e = q[L][lane];
+        __builtin_amdgcn_wave_barrier();
+        qn[L] = 0;
+        run_level<L>(e, pred);
+      }
+      flush<L + 1>();
+    }
+  }
+};
+
+template <int NL, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) sieve_queue_kernel(SieveArgs a) {
+  constexpr int LMAIN = PT.off[NL];
+  constexpr int LFILT = 90;
+  using QS = QSieve<NL, WAVES>;
+  __shared__ uint64_t tab[LMAIN + LFILT];
+  __shared__ QEntry queues[WAVES][QLEVELS - 1][QS::QCAP];
+  for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) tab[i] = a.tab[i];
+  for (int i = threadIdx.x; i < LFILT; i += blockDim.x) tab[LMAIN + i] = a.tab[NOK + i];
+  __syncthreads();
+
+  const int wave = threadIdx.x >> 6;
+  QS qs;
+  qs.lane = threadIdx.x & 63;
+  qs.q = queues[wave];
+  for (int i = 0; i < QLEVELS - 1; i++) qs.qn[i] = 0;
+  qs.ldsb = (const char *)tab;
+  qs.gtb = (const char *)a.tab;
+  qs.a = &a;
+
+  const double S59d = (double)a.kp.S59;
+  const double MODd = (double)AP26_MOD;
+  const double WRAP = MODd + TWO52;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t b = a.blk0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool has = b < a.blk1;
+  double xd = has ? (double)n53_of_block(a.kp, a.n43, b) + TWO52 : TWO52;
+  int i59 = 0;
+  while (__ballot(has) != 0) {
+    uint64_t m = 0;
+    if (has) m = apply_primes<NL, 0, QB[1]>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
+    qs.push(0, xd, m, has && m != 0);
+    if (has) {
+      xd += S59d;
+      if (xd >= WRAP) xd -= MODd;
+      if (++i59 == AP26_N59_STEPS) {
+        i59 = 0;
+        b += stride;
+        has = b < a.blk1;
+        if (has) xd = (double)n53_of_block(a.kp, a.n43, b) + TWO52;
+      }
+    }
+    qs.template drain<0>();
+  }
+  qs.template flush<0>();
+}
+
 /* ---- walk -------------------------------------------------------------- */
 /* For each survivor n: count base-2 SPRPs forward from n+5*STEP; only if the
  * count reaches 10, extend backward from n+4*STEP while the term is positive;

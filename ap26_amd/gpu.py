@@ -40,7 +40,7 @@ class Range(ctypes.Structure):
 
 
 class Opts(ctypes.Structure):
-    _fields_ = [("shifts_per_pass", ctypes.c_int32), ("block", ctypes.c_int32),
+    _fields_ = [("sieve_variant", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("blocks_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32)]
 
 
@@ -124,10 +124,10 @@ def _ptr(a):
 class Context:
     """One ap26_ctx: device buffers and a stream on one GPU."""
 
-    def __init__(self, device=0, shifts_per_pass=0, block=0, blocks_per_cu=0, verbose=0):
+    def __init__(self, device=0, sieve_variant=0, block=0, blocks_per_cu=0, verbose=0):
         lib = load()
         self._lib = lib
-        opts = Opts(shifts_per_pass, block, blocks_per_cu, verbose)
+        opts = Opts(sieve_variant, block, blocks_per_cu, verbose)
         h = ctypes.c_void_p()
         _check(lib.ap26_open(device, ctypes.byref(opts), ctypes.byref(h)), "ap26_open")
         self.h = h

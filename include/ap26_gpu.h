@@ -48,8 +48,9 @@ typedef struct {
 
 /* Tuning knobs (0 = library default / autotuner cache). */
 typedef struct {
-  int32_t shifts_per_pass;  /* 1, 2, 5 or 10 shifts sieved per residue pass */
-  int32_t block;            /* threads per workgroup of the sieve kernel */
+  int32_t sieve_variant;    /* 0 queued-compaction sieve (default), 1 simple
+                               per-shift sieve, 2 simple sieve, 2 shifts per pass */
+  int32_t block;            /* (reported) threads per workgroup of the sieve */
   int32_t blocks_per_cu;    /* resident workgroups per CU (grid = CUs x this) */
   int32_t verbose;
 } ap26_opts;
