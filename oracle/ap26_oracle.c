@@ -378,3 +378,64 @@ uint64_t ap26o_hash_word(int32_t KMIN, int32_t KMAX, int32_t hash) {
   while (minmax > O_MAXINTV) minmax -= O_MAXINTV;
   return (minmax << 32) | (uint32_t)hash;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Algorithmic work count of the sieve: the number of table lookups a mask
+   evaluation performs when every shift's 64-bit mask is evaluated on its own
+   with the reference GPU kernel's early-exit grouping (groups of 5,5,5,4,4
+   primes, then pairs: src/opencl/kernels/sieve.cl:32-114).  Summed over the
+   positions of the span; used to price the sieve against the INT32 roofline. */
+int64_t ap26o_lookup_count(int32_t K, int32_t SHIFT0, int32_t it0, int32_t it1, int32_t j0,
+                           int32_t j1) {
+  pthread_once(&g_once, init_primes);
+  static const int group_end[] = {5, 10, 15, 19, 23, 25, 27, 29, 31, 33, 35, 37, 39, 41, 43, 45, 47,
+                                  49, 51, 53, 55, 57, 59, 61, 63, 65, 67, 69, 71, 73, 75, 77, 79, 81,
+                                  83};
+  const int ngroups = (int)(sizeof(group_end) / sizeof(group_end[0]));
+  ap26o_kconst kc;
+  ap26o_kconsts(K, &kc);
+  int64_t *n43 = (int64_t *)malloc(sizeof(int64_t) * AP26O_NN43);
+  ap26o_n43_table(&kc, n43);
+  int ns = it1 - it0;
+  uint64_t *okok = (uint64_t *)malloc(sizeof(uint64_t) * AP26O_NOK * (size_t)(ns > 0 ? ns : 1));
+  for (int s = 0; s < ns; s++)
+    ap26o_okok_table(K, SHIFT0 + 64 * (it0 + s), okok + (size_t)s * AP26O_NOK);
+  int64_t total = 0;
+  for (int j = j0; j < j1; j++) {
+    int64_t a43 = n43[j];
+    for (int i43 = 0; i43 < 19; i43++) {
+      int64_t a47 = a43;
+      for (int i47 = 0; i47 < 23; i47++) {
+        int64_t a53 = a47;
+        for (int i53 = 0; i53 < 29; i53++) {
+          int64_t n59 = a53;
+          for (int i59 = 0; i59 < 35; i59++) {
+            uint32_t r[AP26O_NPRIMES];
+            for (int i = 0; i < AP26O_NPRIMES; i++) r[i] = mod_small((uint64_t)n59, i);
+            for (int s = 0; s < ns; s++) {
+              const uint64_t *t = okok + (size_t)s * AP26O_NOK;
+              uint64_t m = ~UINT64_C(0);
+              int i = 0;
+              for (int g = 0; g < ngroups; g++) {
+                for (; i < group_end[g]; i++) m &= t[g_offsets[i] + r[i]];
+                total += group_end[g] - (g ? group_end[g - 1] : 0);
+                if (!m) break;
+              }
+            }
+            n59 += kc.S59;
+            if (n59 >= O_MOD) n59 -= O_MOD;
+          }
+          a53 += kc.S53;
+          if (a53 >= O_MOD) a53 -= O_MOD;
+        }
+        a47 += kc.S47;
+        if (a47 >= O_MOD) a47 -= O_MOD;
+      }
+      a43 += kc.S43;
+      if (a43 >= O_MOD) a43 -= O_MOD;
+    }
+  }
+  free(okok);
+  free(n43);
+  return total;
+}

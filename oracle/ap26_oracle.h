@@ -93,6 +93,12 @@ int ap26o_report(int32_t k, int32_t K, int64_t first, int32_t *hash,
 /* validate_ap26 / val_base2_ap26 (src/common/AP26.cpp:484-543). */
 int32_t ap26o_validate(int32_t k, int32_t K, int64_t first, int nbases);
 
+/* Table lookups of the span under the reference GPU kernel's early-exit
+   grouping (src/opencl/kernels/sieve.cl:32-114), each shift's mask on its own:
+   the algorithmic work count the sieve's roofline is priced on. */
+int64_t ap26o_lookup_count(int32_t K, int32_t SHIFT0, int32_t it0, int32_t it1, int32_t j0,
+                           int32_t j1);
+
 /* 64-bit result hash word (src/common/AP26.cpp:295-357). */
 uint64_t ap26o_hash_word(int32_t KMIN, int32_t KMAX, int32_t hash);
 

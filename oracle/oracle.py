@@ -46,6 +46,7 @@ def load():
                                             ctypes.POINTER(I64), ctypes.POINTER(I64)]),
             "ap26o_validate": (I32, [I32, I32, I64, ctypes.c_int]),
             "ap26o_hash_word": (ctypes.c_uint64, [I32, I32, I32]),
+            "ap26o_lookup_count": (I64, [I32, I32, I32, I32, I32, I32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
@@ -156,3 +157,7 @@ def validate(k, K, first, nbases=9):
 
 def hash_word(kmin, kmax, h):
     return load().ap26o_hash_word(kmin, kmax, h)
+
+
+def lookup_count(K, shift, it0, it1, j0, j1):
+    return load().ap26o_lookup_count(K, shift, it0, it1, j0, j1)

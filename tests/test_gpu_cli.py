@@ -1,0 +1,100 @@
+"""The AP26 command (ap26_amd/bin/AP26) as a drop-in for the reference's
+`AP26 KMIN KMAX SHIFT`: SOL file, hash line, state files, resume."""
+import os
+import subprocess
+
+import pytest
+
+from ap26_amd import host as H
+from conftest import GOLDEN, ROOT, read_sol
+
+pytestmark = pytest.mark.gpu
+AP26 = os.path.join(ROOT, "ap26_amd", "bin", "AP26")
+
+
+def run(args, cwd):
+    r = subprocess.run([AP26, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300)
+    return r
+
+
+def golden(golden_index, name):
+    return [g for g in golden_index if g["name"] == name][0]
+
+
+def test_test_file_byte_identical(tmp_path):
+    """README.md:47-51: SOL-AP26.txt of `AP26 366384 366384 0` equals tests/TEST-366384.txt."""
+    r = run([366384, 366384, 0], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "SOL-AP26.txt").read_text() == \
+        open(os.path.join(GOLDEN, "ref_tests", "TEST-366384.txt")).read()
+    states = [(tmp_path / f).read_text().split() for f in ("AP26-state.a.txt", "AP26-state.b.txt")]
+    final = [s for s in states if s[3] == "366385"][0]
+    assert final[:6] == ["366384", "366384", "0", "366385", "0", str(0x15550)]
+
+
+def _units_hash(golden_index, names, kmin, kmax):
+    T = 0
+    recs = []
+    for n in names:
+        rr, hl = read_sol(os.path.join(GOLDEN, "ref_min10", n + ".sol"))
+        recs += rr
+        T += int(hl[8:], 16)
+    return sorted(recs), H.hash_line(kmin, kmax, H.cap_total(T))
+
+
+def test_multi_k_unit_with_skipped_k(golden_index, tmp_path):
+    r = run([875858, 875860, 640, "-report", 10], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "2 K in this range remaining to be searched (1 skipped, 0 done)" in r.stdout
+    want = _units_hash(golden_index, ["c1_875859", "c1_875860"], 875858, 875860)
+    assert read_sol(str(tmp_path / "SOL-AP26.txt")) == want
+
+
+def test_resume_from_reference_style_checkpoint(golden_index, tmp_path):
+    """A state file as the reference writes it after finishing K=875859:
+    the resumed run searches only 875860 and ends with the full unit's hash."""
+    T59 = int(golden(golden_index, "c1_875859")["hash"][8:], 16)
+    (tmp_path / "AP26-state.a.txt").write_text(f"875858 875860 640 875860 0 {T59} 1234.000000\n")
+    r = run([875858, 875860, 640, "-report", 10], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Resuming search from checkpoint." in r.stdout
+    recs, hline = read_sol(str(tmp_path / "SOL-AP26.txt"))
+    want_recs, want_hash = _units_hash(golden_index, ["c1_875859", "c1_875860"], 875858, 875860)
+    assert hline == want_hash
+    assert recs == read_sol(os.path.join(GOLDEN, "ref_min10", "c1_875860.sol"))[0]
+
+
+def test_resume_mid_k_and_shift_checkpoints(golden_index, tmp_path):
+    """-shiftckpt writes ITER = shifts done, as the reference CPU build does; a
+    run resumed from its ITER=4 state ends with the golden hash."""
+    a = tmp_path / "a"
+    a.mkdir()
+    r = run([875861, 875861, 640, "-report", 10, "-shiftckpt"], a)
+    assert r.returncode == 0, r.stderr
+    ck = [l for l in r.stdout.splitlines() if l.startswith("Checkpoint:")]
+    assert [l.split("ITER:")[1] for l in ck] == [str(i) for i in range(10)] + ["0"]
+    g = golden(golden_index, "c1_875861")
+    assert read_sol(str(a / "SOL-AP26.txt")) == read_sol(os.path.join(GOLDEN, "ref_min10", "c1_875861.sol"))
+    # replay: take the hash the run had after 4 shifts, resume from there
+    b = tmp_path / "b"
+    b.mkdir()
+    # the hash after shifts 0..3, from a 4-shift run of the same unit
+    c = tmp_path / "c"
+    c.mkdir()
+    from ap26_amd import gpu
+    with gpu.Context(0) as ctx:
+        sols, _ = ctx.search_span(875861, 640, 0, 4, 0, 10840)
+    hh = H.Host(875861, 875861, 640, str(c), min_report=10)
+    for s in sols.tolist():
+        assert hh.report(s[0], s[1], s[2]) == 0
+    h4 = hh.hash
+    hh.close()
+    (b / "AP26-state.b.txt").write_text(f"875861 875861 640 875861 4 {h4} 1234.000000\n")
+    r = run([875861, 875861, 640, "-report", 10], b)
+    assert r.returncode == 0, r.stderr
+    assert read_sol(str(b / "SOL-AP26.txt"))[1] == g["hash"]
+
+
+def test_bad_arguments(tmp_path):
+    r = run([1], tmp_path)
+    assert r.returncode == 1 and "Usage" in r.stdout

@@ -1,0 +1,122 @@
+"""Host side of the operator surface (libap26host.so), CPU only: reporting and
+hash rules against the reference's own ReportSolution / golden files, and the
+A/B checkpoint state files."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ap26_amd import host as H
+from conftest import GOLDEN, read_sol
+
+
+def test_report_vectors_from_reference(tmp_path):
+    vecs = json.load(open(os.path.join(GOLDEN, "report_vectors.json")))
+    for i, v in enumerate(vecs):
+        d = tmp_path / f"c{i}"
+        d.mkdir()
+        h = H.Host(1, 1, 0, str(d), min_report=20)
+        rc = 0
+        for (k, K, first), want in zip(v["reports"], v["hashes"] + [None] * 9):
+            rc = h.report(k, K, first)
+            if rc:
+                break
+            assert h.hash == want
+        h.close()
+        assert (rc == H.AP26H_E_INVALID) == (v["exit"] != 0)
+        sol = d / "SOL-AP26.txt"
+        lines = sol.read_text().splitlines() if sol.exists() else []
+        if rc == 0:
+            assert lines == v["sol_lines"]
+            assert h.st.n_nonsol == sum(len(x) for x in [v["nonsolutions"]])
+
+
+def test_hash_lines_of_golden_units(golden_index):
+    for g in golden_index:
+        recs, hline = read_sol(os.path.join(GOLDEN, "ref_min10", g["name"] + ".sol"))
+        total = sum(f % 1000 + k for k, _, f in recs)
+        assert H.hash_line(g["kmin"], g["kmax"], H.cap_total(total)) == hline
+
+
+def test_cap_total_equals_sequential_rule():
+    """result_hash += x; if (result_hash > 2e9) result_hash -= 2e9 (AP26.cpp:557-561)."""
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        xs = rng.integers(0, 1_000_000_000, size=int(rng.integers(0, 40))).tolist()
+        if rng.random() < 0.2:
+            xs = [0] * len(xs)
+        h = 0
+        for x in xs:
+            h += x
+            if h > 2_000_000_000:
+                h -= 2_000_000_000
+        assert H.cap_total(sum(xs)) == h
+
+
+def test_hash_word_range_fold():
+    # KMIN+KMAX above 2e9 is folded (AP26.cpp:310-312)
+    assert H.hash_word(1_500_000_000, 1_500_000_001, 5) == ((3_000_000_001 - 2_000_000_000) << 32) | 5
+    assert H.hash_line(366384, 366384, 0x15550) == "000B2E6000015550"
+
+
+def test_write_hash_appends_line(tmp_path):
+    h = H.Host(366384, 366384, 0, str(tmp_path))
+    assert h.report(25, 366384, 6171054912832631) == 0
+    h.st.result_hash = 0x15550
+    assert h.write_hash() == 0
+    assert (tmp_path / "SOL-AP26.txt").read_text() == open(
+        os.path.join(GOLDEN, "ref_tests", "TEST-366384.txt")).read()
+
+
+def test_state_round_trip_and_alternation(tmp_path):
+    h = H.Host(100, 200, 640, str(tmp_path))
+    h.st.last_trickle = 1234.0
+    h.st.result_hash = 77
+    h.write_state(150, 0)   # -> A
+    assert (tmp_path / "AP26-state.a.txt").read_text() == "100 200 640 150 0 77 1234.000000\n"
+    h.st.result_hash = 99
+    h.write_state(151, 4)   # -> B
+    assert (tmp_path / "AP26-state.b.txt").read_text() == "100 200 640 151 4 99 1234.000000\n"
+    r = H.Host(100, 200, 640, str(tmp_path))
+    assert r.read_state() == (151, 4) and r.hash == 99
+    assert r.st.write_state_a_next == 1   # B was newest -> write A next
+    # newest by ITER at equal K
+    h.write_state(151, 8)   # -> A
+    r = H.Host(100, 200, 640, str(tmp_path))
+    assert r.read_state() == (151, 8) and r.st.write_state_a_next == 0
+
+
+def test_state_mismatch_and_corruption(tmp_path):
+    h = H.Host(100, 200, 640, str(tmp_path))
+    h.write_state(150, 0)   # A
+    h.write_state(160, 0)   # B
+    # another unit's state is ignored
+    assert H.Host(100, 200, 0, str(tmp_path)).read_state() is None
+    # a corrupt newest file falls back to the other
+    (tmp_path / "AP26-state.b.txt").write_text("garbage\n")
+    assert H.Host(100, 200, 640, str(tmp_path)).read_state() == (150, 0)
+    (tmp_path / "AP26-state.a.txt").unlink()
+    assert H.Host(100, 200, 640, str(tmp_path)).read_state() is None
+
+
+def test_reference_final_state_format(golden_index):
+    """The final state line of a reference run: KMIN KMAX SHIFT KMAX+1 0 HASH TRICKLE."""
+    for g in golden_index:
+        states = g["state_files"]
+        final = [v for v in states.values() if v.split()[3] == str(g["kmax"] + 1)]
+        assert final, g["name"]
+        f = final[0].split()
+        assert f[:5] == [str(g["kmin"]), str(g["kmax"]), str(g["shift"]), str(g["kmax"] + 1), "0"]
+        assert int(f[5]) == int(g["hash"][8:], 16)
+
+
+def test_count():
+    h = H.Host(875858, 875873, 640)
+    assert h.count(875858) == (13, 0)
+    assert h.count(875866) == (13, 6)
+
+
+def test_validate_range_error():
+    j, err = H.validate(2, 1, (1 << 62) + 1, 1)
+    assert err == H.AP26H_E_RANGE
