@@ -45,9 +45,9 @@ SieveVariant make_variant() {
   return SieveVariant{S, NL, 256, &sieve_kernel<S, NL>};
 }
 
-template <int NL, int WAVES>
+template <int NL, int WAVES, class LV, bool VC>
 SieveVariant make_qvariant() {
-  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES>};
+  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC>};
 }
 
 }  // namespace
@@ -70,12 +70,25 @@ struct ap26_ctx {
   SieveVariant sieve{};
 };
 
-/* variant ids: 0 = queued sieve (default), 1 = simple S=1, 2 = simple S=2 */
+/* Sieve variants: 0 = default queued sieve, 1 = simple S=1, 2 = simple S=2,
+   10+ = queued-sieve tuning candidates (LDS primes, waves per CU, levels, VGPR
+   constants). */
+#define AP26_NVARIANTS 20
 static SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
     case 2: return make_variant<2, 45>();
-    default: return make_qvariant<57, 8>();
+    case 10: return make_qvariant<57, 8, Levels4, false>();
+    case 11: return make_qvariant<57, 8, Levels4, true>();
+    case 12: return make_qvariant<61, 8, Levels4, true>();
+    case 13: return make_qvariant<45, 12, Levels4, false>();
+    case 14: return make_qvariant<57, 16, Levels3, false>();
+    case 15: return make_qvariant<45, 16, Levels3, false>();
+    case 16: return make_qvariant<45, 8, Levels5, true>();
+    case 17: return make_qvariant<65, 8, Levels3, true>();
+    case 18: return make_qvariant<45, 12, Levels3, false>();
+    case 19: return make_qvariant<31, 16, Levels4, false>();
+    default: return make_qvariant<57, 8, Levels4, true>();
   }
 }
 

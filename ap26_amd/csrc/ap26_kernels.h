@@ -227,30 +227,84 @@ __global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
  * and writes the survivors out.  Levels have no internal exits.  Tables of the
  * first NL primes live in LDS, the rest are read through L1/L2. */
 
-/* level boundaries (prime indices): [0,15) 61..131, [15,29) 137..199,
-   [29,45) 211..307, [45,83) 311..541 */
-constexpr int QLEVELS = 4;
-constexpr int QB[QLEVELS + 1] = {0, 15, 29, 45, 83};
+/* Level sets: prime-index boundaries of the levels. */
+template <int... Bs>
+struct Levels {
+  static constexpr int N = sizeof...(Bs) - 1; /* levels; N-1 queues */
+  static constexpr int B[sizeof...(Bs)] = {Bs...};
+};
+using Levels4 = Levels<0, 15, 29, 45, 83>;     /* 61..131 | 137..199 | 211..307 | 311..541 */
+using Levels3 = Levels<0, 15, 29, 83>;
+using Levels5 = Levels<0, 12, 20, 30, 45, 83>;
 
 struct QEntry {
   double xd;  /* n59 + 2^52 */
   uint64_t m; /* mask so far */
 };
 
+/* Per-lane copies of the level-0 constants held in VGPRs: with ~45 scalar
+   constants live the compiler otherwise recycles one SGPR pair for all of them
+   and serialises the lookups. */
+template <int B, int E>
+struct VConsts {
+  double inv[E - B];
+  int m8p[E - B];
+  __device__ __forceinline__ void init() {
+    static_for<B, E>([&](auto IC) {
+      constexpr int I = decltype(IC)::value;
+      constexpr double c = 1.0 / (double)PT.p[I];
+      asm("v_mov_b64 %0, %1" : "=v"(inv[I - B]) : "s"(c));
+      asm("v_mov_b32 %0, %1" : "=v"(m8p[I - B]) : "s"(-8 * PT.p[I]));
+    });
+  }
+};
+
 template <int NL, int B, int E>
 __device__ __forceinline__ uint64_t apply_primes(uint64_t m, double xd, const char *ldsb, const char *gtb) {
   const double x = xd - TWO52;
   const uint32_t xs = lo32(xd) * 8u;
+  int bo[E - B];
+  uint64_t w[E - B];
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    bo[I - B] = centred_byteoff<PT.p[I], 8>(x, xs);
+  });
   static_for<B, E>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     constexpr int P = PT.p[I];
-    const int bo = centred_byteoff<P, 8>(x, xs);
     constexpr int C0 = 8 * (PT.off[I] + (P - 1) / 2);
     if constexpr (I < NL)
-      m &= *(const uint64_t *)(ldsb + C0 + bo);
+      w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
     else
-      m &= *(const uint64_t *)(gtb + (int64_t)(C0 + bo));
+      w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
   });
+  static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
+  return m;
+}
+
+/* same, with the constants in VGPRs (all primes in LDS); offsets, loads and
+   ANDs are issued as three batches so the LDS reads overlap */
+template <int B, int E>
+__device__ __forceinline__ uint64_t apply_primes_vc(uint64_t m, double xd, const char *ldsb,
+                                                    const VConsts<B, E> &vc) {
+  const double x = xd - TWO52;
+  const uint32_t xs = lo32(xd) * 8u;
+  int bo[E - B];
+  uint64_t w[E - B];
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    const double qd = __fma_rn(x, vc.inv[I - B], SHIFTER);
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"((int)lo32(qd)), "v"(vc.m8p[I - B]), "v"(xs));
+    bo[I - B] = (r << 5) >> 5;
+  });
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    constexpr int P = PT.p[I];
+    constexpr int C0 = 8 * (PT.off[I] + (P - 1) / 2);
+    w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
+  });
+  static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
   return m;
 }
 
@@ -293,11 +347,12 @@ __device__ __forceinline__ void emit_survivors(uint64_t w, double xd, const Siev
   }
 }
 
-template <int NL, int WAVES>
+template <int NL, class LV>
 struct QSieve {
   static constexpr int QCAP = 128;
-  QEntry (*q)[QCAP];  /* this wave's QLEVELS-1 queues */
-  int qn[QLEVELS - 1];
+  static constexpr int NQ = LV::N - 1;
+  QEntry (*q)[QCAP];  /* this wave's NQ queues */
+  int qn[NQ];
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
@@ -314,8 +369,8 @@ struct QSieve {
   template <int L>
   __device__ __forceinline__ void run_level(const QEntry &e, bool pred) {
     uint64_t m = e.m;
-    if (pred) m = apply_primes<NL, QB[L + 1], QB[L + 2]>(m, e.xd, ldsb, gtb);
-    if constexpr (L + 2 == QLEVELS) {
+    if (pred) m = apply_primes<NL, LV::B[L + 1], LV::B[L + 2]>(m, e.xd, ldsb, gtb);
+    if constexpr (L + 2 == LV::N) {
       if (pred && m) m = apply_filters<NL>(m, e.xd, ldsb);
       emit_survivors(pred ? m : 0, e.xd, *a, lane);
     } else {
@@ -338,7 +393,7 @@ struct QSieve {
   /* flush the partial queues, lowest level first */
   template <int L>
   __device__ __forceinline__ void flush() {
-    if constexpr (L < QLEVELS - 1) {
+    if constexpr (L < NQ) {
       drain<L>();
       const int n = qn[L];
       if (n > 0) {
@@ -354,13 +409,22 @@ struct QSieve {
   }
 };
 
-template <int NL, int WAVES>
-__global__ void __launch_bounds__(64 * WAVES) sieve_queue_kernel(SieveArgs a) {
+template <int NL, int WAVES, class LV>
+constexpr int qsieve_lds_bytes() {
+  return (PT.off[NL] + 90) * 8 + WAVES * (LV::N - 1) * 128 * (int)sizeof(QEntry);
+}
+
+/* VC: keep the level-0 constants in VGPRs */
+template <int NL, int WAVES, class LV, bool VC>
+__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+sieve_queue_kernel(SieveArgs a) {
+  static_assert(qsieve_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
   constexpr int LMAIN = PT.off[NL];
   constexpr int LFILT = 90;
-  using QS = QSieve<NL, WAVES>;
+  using QS = QSieve<NL, LV>;
   __shared__ uint64_t tab[LMAIN + LFILT];
-  __shared__ QEntry queues[WAVES][QLEVELS - 1][QS::QCAP];
+  __shared__ QEntry queues[WAVES][QS::NQ][QS::QCAP];
   for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) tab[i] = a.tab[i];
   for (int i = threadIdx.x; i < LFILT; i += blockDim.x) tab[LMAIN + i] = a.tab[NOK + i];
   __syncthreads();
@@ -369,10 +433,12 @@ __global__ void __launch_bounds__(64 * WAVES) sieve_queue_kernel(SieveArgs a) {
   QS qs;
   qs.lane = threadIdx.x & 63;
   qs.q = queues[wave];
-  for (int i = 0; i < QLEVELS - 1; i++) qs.qn[i] = 0;
+  for (int i = 0; i < QS::NQ; i++) qs.qn[i] = 0;
   qs.ldsb = (const char *)tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
+  VConsts<0, LV::B[1]> vc;
+  if constexpr (VC) vc.init();
 
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
@@ -384,7 +450,12 @@ __global__ void __launch_bounds__(64 * WAVES) sieve_queue_kernel(SieveArgs a) {
   int i59 = 0;
   while (__ballot(has) != 0) {
     uint64_t m = 0;
-    if (has) m = apply_primes<NL, 0, QB[1]>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
+    if (has) {
+      if constexpr (VC)
+        m = apply_primes_vc<0, LV::B[1]>(~(uint64_t)0, xd, qs.ldsb, vc);
+      else
+        m = apply_primes<NL, 0, LV::B[1]>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
+    }
     qs.push(0, xd, m, has && m != 0);
     if (has) {
       xd += S59d;
