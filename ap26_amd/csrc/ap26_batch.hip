@@ -1,0 +1,311 @@
+/* ap26_batch.hip -- the batched engine behind ap26_search_batch: many work
+ * units (KMIN..KMAX, SHIFT, resumed shift) expanded into K jobs and searched
+ * in rounds of up to `per_round` K.  One round is a captured hipGraph
+ *   n43 starts -> tables -> persistent work-stealing sieve -> walk -> advance
+ * whose kernels read the round number from device memory, so the same graph
+ * is replayed for every round with no host synchronisation in between.
+ * Reports carry their range id; per-range hash totals are accumulated on the
+ * device as exact 64-bit sums.
+ */
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "ap26_ctx.h"
+#include "ap26_host_math.h"
+
+using namespace ap26;
+
+namespace {
+
+constexpr int BATCH_SOL_CAP = 1 << 20;
+constexpr int MAX_RANGES = 1 << 20;
+
+using PersistFn = void (*)(PersistArgs);
+
+struct PersistVariant {
+  int NL, waves;
+  PersistFn fn;
+};
+
+template <int NL, int WAVES, class LV>
+PersistVariant make_pvariant() {
+  return PersistVariant{NL, WAVES, &sieve_persistent_kernel<NL, WAVES, LV>};
+}
+
+PersistVariant pick_pvariant(int id) {
+  switch (id) {
+    case 1: return make_pvariant<31, 16, Levels4>();
+    case 2: return make_pvariant<45, 12, Levels4>();
+    case 3: return make_pvariant<25, 16, Levels5>();
+    default: return make_pvariant<38, 16, Levels4>();
+  }
+}
+
+}  // namespace
+
+struct BatchState {
+  int per_round = 0;
+  int chunk = 0, nchunk = 0;
+  int pvariant = -1;
+  PersistVariant pv{};
+  int64_t surv_cap = 0;
+  int max_jobs = 0;          /* capacity of d_jobs */
+  int max_ranges = 0;
+  BatchJobDev *d_jobs = nullptr;
+  RoundState *d_rs = nullptr;
+  uint32_t *d_triples = nullptr;
+  int64_t *d_n43 = nullptr;
+  uint64_t *d_tab = nullptr;
+  int64_t *d_surv = nullptr;
+  int32_t *d_surv_job = nullptr;
+  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) */
+  DevSol *d_sols = nullptr;
+  unsigned long long *d_hash = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_njobs = -1;      /* graph is valid for this per_round/chunk setup */
+};
+
+void batch_free(ap26_ctx *ctx) {
+  BatchState *b = ctx->batch;
+  if (!b) return;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  (void)hipFree(b->d_jobs);
+  (void)hipFree(b->d_rs);
+  (void)hipFree(b->d_triples);
+  (void)hipFree(b->d_n43);
+  (void)hipFree(b->d_tab);
+  (void)hipFree(b->d_surv);
+  (void)hipFree(b->d_surv_job);
+  (void)hipFree(b->d_cnt);
+  (void)hipFree(b->d_sols);
+  (void)hipFree(b->d_hash);
+  delete b;
+  ctx->batch = nullptr;
+}
+
+static int batch_init(ap26_ctx *ctx) {
+  if (ctx->batch) return AP26_OK;
+  BatchState *b = new (std::nothrow) BatchState();
+  if (!b) return AP26_E_NOMEM;
+  ctx->batch = b;
+  const char *e = getenv("AP26_BATCH_ROUND");
+  b->per_round = e ? atoi(e) : 16;
+  if (b->per_round < 1 || b->per_round > 64) b->per_round = 16;
+  e = getenv("AP26_BATCH_CHUNK");
+  b->chunk = e ? atoi(e) : 85;
+  if (b->chunk < 1 || b->chunk > AP26_NN43) b->chunk = 85;
+  b->nchunk = (AP26_NN43 + b->chunk - 1) / b->chunk;
+  e = getenv("AP26_BATCH_VARIANT");
+  b->pvariant = e ? atoi(e) : 0;
+  b->pv = pick_pvariant(b->pvariant);
+  b->surv_cap = (int64_t)b->per_round * (INT64_C(60) << 20); /* ~45M survivors per K */
+  /* guarded (i31, i37, i41) triples in the reference order */
+  std::vector<uint32_t> trip;
+  for (int i31 = 0; i31 < 7; i31++)
+    for (int i37 = 0; i37 < 13; i37++) {
+      if (i37 - i31 > 10 || i31 - i37 > 4) continue;
+      for (int i41 = 0; i41 < 17; i41++) {
+        if (i41 - i31 > 14 || i41 - i37 > 14 || i31 - i41 > 4 || i37 - i41 > 10) continue;
+        trip.push_back((uint32_t)i31 | ((uint32_t)i37 << 8) | ((uint32_t)i41 << 16));
+      }
+    }
+  if (trip.size() * 8 != AP26_NN43) return AP26_E_INTERNAL;
+  if (hipMalloc(&b->d_rs, sizeof(RoundState)) != hipSuccess ||
+      hipMalloc(&b->d_triples, trip.size() * 4) != hipSuccess ||
+      hipMalloc(&b->d_n43, sizeof(int64_t) * AP26_NN43 * b->per_round) != hipSuccess ||
+      hipMalloc(&b->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT * b->per_round) != hipSuccess ||
+      hipMalloc(&b->d_surv, sizeof(int64_t) * b->surv_cap) != hipSuccess ||
+      hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess ||
+      hipMalloc(&b->d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
+      hipMalloc(&b->d_sols, sizeof(DevSol) * BATCH_SOL_CAP) != hipSuccess)
+    return AP26_E_NOMEM;
+  HIPCHK(hipMemcpy(b->d_triples, trip.data(), trip.size() * 4, hipMemcpyHostToDevice));
+  return AP26_OK;
+}
+
+static int ensure_capacity(ap26_ctx *ctx, int njobs, int nranges) {
+  BatchState *b = ctx->batch;
+  if (njobs > b->max_jobs) {
+    (void)hipFree(b->d_jobs);
+    b->d_jobs = nullptr;
+    int cap = njobs < 1024 ? 1024 : njobs;
+    if (hipMalloc(&b->d_jobs, sizeof(BatchJobDev) * cap) != hipSuccess) return AP26_E_NOMEM;
+    b->max_jobs = cap;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  if (nranges > b->max_ranges) {
+    (void)hipFree(b->d_hash);
+    b->d_hash = nullptr;
+    int cap = nranges < 1024 ? 1024 : nranges;
+    if (hipMalloc(&b->d_hash, sizeof(unsigned long long) * cap) != hipSuccess) return AP26_E_NOMEM;
+    b->max_ranges = cap;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  return AP26_OK;
+}
+
+/* enqueue one round on ctx->stream (captured into the graph) */
+static int enqueue_round(ap26_ctx *ctx) {
+  BatchState *b = ctx->batch;
+  const int R = b->per_round;
+  hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, b->d_jobs,
+                     b->d_rs, b->d_triples, b->d_n43);
+  hipLaunchKernelGGL(batch_tables_kernel, dim3((541 + 63) / 64, NPT, R * AP26_NSHIFT), dim3(64), 0,
+                     ctx->stream, b->d_jobs, b->d_rs, b->d_tab);
+  PersistArgs p;
+  p.jobs = b->d_jobs;
+  p.rs = b->d_rs;
+  p.n43 = b->d_n43;
+  p.tab = b->d_tab;
+  p.surv = b->d_surv;
+  p.surv_job = b->d_surv_job;
+  p.nsurv = b->d_cnt + 0;
+  p.cap = b->surv_cap;
+  p.work = (unsigned int *)(b->d_cnt + 3);
+  p.chunk = b->chunk;
+  p.nchunk = b->nchunk;
+  hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
+  BatchWalkArgs w;
+  w.jobs = b->d_jobs;
+  w.rs = b->d_rs;
+  w.surv = b->d_surv;
+  w.surv_job = b->d_surv_job;
+  w.nsurv = b->d_cnt + 0;
+  w.cap = b->surv_cap;
+  w.sols = b->d_sols;
+  w.nsol = (unsigned int *)(b->d_cnt + 2);
+  w.sol_cap = BATCH_SOL_CAP;
+  w.ntests = b->d_cnt + 1;
+  w.hash_tot = b->d_hash;
+  hipLaunchKernelGGL(batch_walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+  hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_rs, b->d_cnt + 0,
+                     (unsigned int *)(b->d_cnt + 3));
+  return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
+}
+
+__global__ void batch_reset_kernel(RoundState *rs, int njobs, int per_round, unsigned long long *cnt,
+                                   unsigned long long *hash, int nranges) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    rs->base = 0;
+    rs->njobs = njobs;
+    rs->per_round = per_round;
+    rs->pad = 0;
+  }
+  if (i < 8) cnt[i] = 0;
+  if (i < nranges) hash[i] = 0;
+}
+
+extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out, int32_t cap,
+                                 int32_t *n_out, int64_t *hash_totals, ap26_stats *stats) {
+  if (!ctx || !n_out || nr < 0 || (nr > 0 && !r) || (cap > 0 && !out)) return AP26_E_RANGE;
+  if (nr > MAX_RANGES) return AP26_E_RANGE;
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(ctx->device));
+  *n_out = 0;
+  ap26_stats st{};
+  /* expand units into searched K */
+  std::vector<BatchJobDev> jobs;
+  for (int i = 0; i < nr; i++) {
+    const ap26_range &u = r[i];
+    if (u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || u.shift < 0 || u.shift > (1 << 20) ||
+        u.iter_start < 0 || u.iter_start > AP26_NSHIFT)
+      return AP26_E_RANGE;
+    for (int64_t K = u.kmin; K <= u.kmax; K++) {
+      if (!host_will_search((int32_t)K)) {
+        st.k_skipped++;
+        continue;
+      }
+      BatchJobDev j;
+      HostK hk;
+      host_kconsts((int32_t)K, &hk);
+      j.kp.STEP = hk.STEP;
+      j.kp.n0 = hk.n0;
+      j.kp.S31 = hk.S[0];
+      j.kp.S37 = hk.S[1];
+      j.kp.S41 = hk.S[2];
+      j.kp.S43 = hk.S[3];
+      j.kp.S47 = hk.S[4];
+      j.kp.S53 = hk.S[5];
+      j.kp.S59 = hk.S[6];
+      j.kp.K = (int32_t)K;
+      j.kp.SHIFT = u.shift;
+      j.it0 = (K == u.kmin) ? u.iter_start : 0;
+      j.range_id = i;
+      jobs.push_back(j);
+    }
+  }
+  st.k_searched = (int64_t)jobs.size();
+  if (hash_totals)
+    for (int i = 0; i < nr; i++) hash_totals[i] = 0;
+  if (jobs.empty()) {
+    if (stats) *stats = st;
+    return AP26_OK;
+  }
+  int rc = batch_init(ctx);
+  if (rc) return rc;
+  BatchState *b = ctx->batch;
+  rc = ensure_capacity(ctx, (int)jobs.size(), nr);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(BatchJobDev) * jobs.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  const int nres = nr > 8 ? nr : 8;
+  hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs,
+                     (int)jobs.size(), b->per_round, b->d_cnt, b->d_hash, nr);
+  HIPCHK(hipGetLastError());
+  if (!b->graph) {
+    hipGraph_t g;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_round(ctx);
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || ce != hipSuccess) return AP26_E_DEVICE;
+    hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return AP26_E_DEVICE;
+  }
+  const int rounds = (int)((jobs.size() + b->per_round - 1) / b->per_round);
+  HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+  for (int i = 0; i < rounds; i++) HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
+  unsigned long long cnt[8];
+  HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<unsigned long long> tot(nr);
+  HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const unsigned int nsol = (unsigned int)(cnt[2] & 0xffffffffu);
+  if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
+  std::vector<DevSol> hs(nsol);
+  if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+  const int ncopy = (int)nsol < cap ? (int)nsol : cap;
+  for (int i = 0; i < ncopy; i++) {
+    out[i].k = hs[i].k;
+    out[i].K = hs[i].K;
+    out[i].first = hs[i].first;
+    out[i].range_id = hs[i].range_id;
+    out[i].shift_iter = hs[i].shift_iter;
+  }
+  *n_out = (int32_t)nsol;
+  if (hash_totals)
+    for (int i = 0; i < nr; i++) hash_totals[i] = (int64_t)tot[i];
+  st.prp_tests = (int64_t)cnt[1];
+  st.reports = nsol;
+  st.ms_sieve = 0;
+  {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) st.ms_sieve = ms;
+  }
+  st.sieve_launches = rounds;
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+}

@@ -1,0 +1,77 @@
+/* ap26_ctx.h -- library-private: the ap26_ctx object and helpers shared by
+ * the per-K path (ap26_gpu.hip) and the batched engine (ap26_batch.hip).
+ */
+#ifndef AP26_CTX_H
+#define AP26_CTX_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "../../include/ap26_gpu.h"
+#include "ap26_kernels.h"
+
+using namespace ap26;
+
+#define HIPCHK(x)                                                                   \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) {                                                         \
+      if (ctx && ctx->opts.verbose)                                                 \
+        fprintf(stderr, "ap26: %s failed: %s\n", #x, hipGetErrorString(e_));       \
+      return AP26_E_DEVICE;                                                         \
+    }                                                                               \
+  } while (0)
+
+
+constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* 67M survivors (~39M per K) */
+constexpr int SOL_CAP = 1 << 16;
+constexpr int NCOUNT = 8; /* device counters */
+enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2 };
+
+using SieveFn = void (*)(SieveArgs);
+
+struct SieveVariant {
+  int S, NL, block;
+  SieveFn fn;
+};
+
+template <int S, int NL>
+SieveVariant make_variant() {
+  return SieveVariant{S, NL, 256, &sieve_kernel<S, NL>};
+}
+
+template <int NL, int WAVES, class LV, bool VC>
+SieveVariant make_qvariant() {
+  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC>};
+}
+
+
+struct BatchState;
+struct ap26_ctx {
+  int device = 0;
+  ap26_opts opts{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  int cu_count = 0;
+  size_t lds_per_block = 0;
+  size_t global_mem = 0;
+  char name[256] = {0};
+  /* device buffers */
+  uint64_t *d_tab = nullptr;     /* 10 shifts of tables */
+  int64_t *d_n43 = nullptr;      /* [10840] */
+  int64_t *d_surv = nullptr;     /* SURV_CAP */
+  unsigned long long *d_cnt = nullptr; /* NCOUNT counters (sol count stored as u32 at C_NSOL) */
+  DevSol *d_sols = nullptr;      /* SOL_CAP */
+  SieveVariant sieve{};
+  BatchState *batch = nullptr;
+};
+
+
+SieveVariant pick_variant(int id);
+
+/* batched engine state (ap26_batch.hip) */
+struct BatchState;
+void batch_free(ap26_ctx *ctx);
+
+#endif

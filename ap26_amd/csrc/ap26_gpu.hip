@@ -16,65 +16,13 @@
 
 using namespace ap26;
 
-#define HIPCHK(x)                                                                   \
-  do {                                                                              \
-    hipError_t e_ = (x);                                                            \
-    if (e_ != hipSuccess) {                                                         \
-      if (ctx && ctx->opts.verbose)                                                 \
-        fprintf(stderr, "ap26: %s failed: %s\n", #x, hipGetErrorString(e_));       \
-      return AP26_E_DEVICE;                                                         \
-    }                                                                               \
-  } while (0)
-
-namespace {
-
-constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* 67M survivors (~39M per K) */
-constexpr int SOL_CAP = 1 << 16;
-constexpr int NCOUNT = 8; /* device counters */
-enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2 };
-
-using SieveFn = void (*)(SieveArgs);
-
-struct SieveVariant {
-  int S, NL, block;
-  SieveFn fn;
-};
-
-template <int S, int NL>
-SieveVariant make_variant() {
-  return SieveVariant{S, NL, 256, &sieve_kernel<S, NL>};
-}
-
-template <int NL, int WAVES, class LV, bool VC>
-SieveVariant make_qvariant() {
-  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC>};
-}
-
-}  // namespace
-
-struct ap26_ctx {
-  int device = 0;
-  ap26_opts opts{};
-  hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
-  int cu_count = 0;
-  size_t lds_per_block = 0;
-  size_t global_mem = 0;
-  char name[256] = {0};
-  /* device buffers */
-  uint64_t *d_tab = nullptr;     /* 10 shifts of tables */
-  int64_t *d_n43 = nullptr;      /* [10840] */
-  int64_t *d_surv = nullptr;     /* SURV_CAP */
-  unsigned long long *d_cnt = nullptr; /* NCOUNT counters (sol count stored as u32 at C_NSOL) */
-  DevSol *d_sols = nullptr;      /* SOL_CAP */
-  SieveVariant sieve{};
-};
+#include "ap26_ctx.h"
 
 /* Sieve variants: 0 = default queued sieve, 1 = simple S=1, 2 = simple S=2,
    10+ = queued-sieve tuning candidates (LDS primes, waves per CU, levels, VGPR
    constants). */
-#define AP26_NVARIANTS 20
-static SieveVariant pick_variant(int id) {
+#define AP26_NVARIANTS 26
+SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
     case 2: return make_variant<2, 45>();
@@ -88,7 +36,13 @@ static SieveVariant pick_variant(int id) {
     case 17: return make_qvariant<65, 8, Levels3, true>();
     case 18: return make_qvariant<45, 12, Levels3, false>();
     case 19: return make_qvariant<31, 16, Levels4, false>();
-    default: return make_qvariant<57, 8, Levels4, true>();
+    case 20: return make_qvariant<29, 16, Levels4, false>();
+    case 21: return make_qvariant<35, 16, Levels4, false>();
+    case 22: return make_qvariant<20, 16, Levels5, false>();
+    case 23: return make_qvariant<38, 16, Levels4, false>();
+    case 24: return make_qvariant<25, 16, Levels5, false>();
+    case 25: return make_qvariant<31, 12, Levels5, false>();
+    default: return make_qvariant<38, 16, Levels4, false>();
   }
 }
 
@@ -159,6 +113,7 @@ int ap26_close(ap26_ctx *ctx) {
   (void)hipFree(ctx->d_surv);
   (void)hipFree(ctx->d_cnt);
   (void)hipFree(ctx->d_sols);
+  batch_free(ctx);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -211,6 +166,8 @@ static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int
     a.surv = ctx->d_surv;
     a.nsurv = ctx->d_cnt + C_NSURV;
     a.cap = SURV_CAP;
+    a.surv_job = nullptr;
+    a.job = 0;
     hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
@@ -414,12 +371,6 @@ int ap26_prp_batch(ap26_ctx *ctx, const uint64_t *n, uint8_t *out, size_t count)
   (void)hipFree(dn);
   (void)hipFree(dout);
   return rc;
-}
-
-int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out, int32_t cap,
-                      int32_t *n_out, int64_t *hash_totals, ap26_stats *stats) {
-  (void)ctx; (void)r; (void)nr; (void)out; (void)cap; (void)n_out; (void)hash_totals; (void)stats;
-  return AP26_E_RANGE; /* provided by the batched engine */
 }
 
 int ap26_dev_alloc(ap26_ctx *ctx, size_t bytes, void **dptr) {

@@ -45,18 +45,8 @@ __device__ __forceinline__ void static_for(F &&f) {
  * For the filter primes q = 7..23 bit jj = ((r + (jj+SHIFT)*MOD) mod q != 0),
  * the reference's per-candidate n%7..n%23 test (src/cpu/cpusse2.cpp:572-577).
  * Grid: x = centred residue chunk (64), y = table index, z = shift slot. */
-__global__ void tables_kernel(KParams kp, int32_t it0, int32_t it1, int32_t S,
-                              uint64_t *__restrict__ tab) {
-  const int ti = blockIdx.y;
-  const int slot = blockIdx.z;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ uint64_t table_word(const KParams &kp, int ti, int c, int shift) {
   const int p = PT.p[ti];
-  if (c >= p) return;
-  const int pass = slot / S, s = slot % S;
-  const int it = it0 + slot;
-  uint64_t *dst = tab + (size_t)pass * NTAB * S + (size_t)(PT.off[ti] + c) * S + s;
-  if (it >= it1) { *dst = 0; return; }
-  const int shift = kp.SHIFT + 64 * it;
   const int r = (c + (p + 1) / 2) % p; /* centred entry c holds residue c-(p-1)/2 */
   const int mp = (int)(AP26_MOD % p);
   const bool sieve = ti < NSP;
@@ -78,13 +68,25 @@ __global__ void tables_kernel(KParams kp, int32_t it0, int32_t it1, int32_t S,
   for (int jj = 0; jj < 64; jj++) {
     int v = (r + js * mp) % p;
     bool ok;
-    if (!sieve) ok = v != 0;
-    else if (sp == 0) ok = v != 0;
+    if (!sieve || sp == 0) ok = v != 0;
     else ok = ((p - v) * inv) % p > 23;
     w |= (uint64_t)ok << jj;
     if (++js == p) js = 0;
   }
-  *dst = w;
+  return w;
+}
+
+static __global__ void tables_kernel(KParams kp, int32_t it0, int32_t it1, int32_t S,
+                              uint64_t *__restrict__ tab) {
+  const int ti = blockIdx.y;
+  const int slot = blockIdx.z;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = PT.p[ti];
+  if (c >= p) return;
+  const int pass = slot / S, s = slot % S;
+  const int it = it0 + slot;
+  uint64_t *dst = tab + (size_t)pass * NTAB * S + (size_t)(PT.off[ti] + c) * S + s;
+  *dst = it >= it1 ? 0 : table_word(kp, ti, c, kp.SHIFT + 64 * it);
 }
 
 /* ---- sieve ------------------------------------------------------------- */
@@ -99,6 +101,8 @@ struct SieveArgs {
   int64_t *__restrict__ surv;
   unsigned long long *__restrict__ nsurv;
   int64_t cap;
+  int32_t *__restrict__ surv_job; /* batched engine: job index of each survivor */
+  int32_t job;
 };
 
 /* n53 = (n43[j] + i43*S43 + i47*S47 + i53*S53) mod MOD for block
@@ -128,7 +132,7 @@ __host__ __device__ constexpr bool check_after(int i) { return i == 14 || (i > 1
  * of the 7..23 filter are appended per wave with one atomic.  The reference's
  * sieve (src/opencl/kernels/sieve.cl:28-152, src/cpu/cpusse2.cpp:410-614). */
 template <int S, int NL>
-__global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
+static __global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
   constexpr int LMAIN = PT.off[NL] * S;
   constexpr int LFILT = 90 * S;
   __shared__ uint64_t lds[LMAIN + LFILT];
@@ -342,7 +346,10 @@ __device__ __forceinline__ void emit_survivors(uint64_t w, double xd, const Siev
       const int bit = __ffsll((unsigned long long)w) - 1;
       w &= w - 1;
       const unsigned long long idx = base + (unsigned long long)pre;
-      if (idx < (unsigned long long)a.cap) a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
+      if (idx < (unsigned long long)a.cap) {
+        a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
+        if (a.surv_job) a.surv_job[idx] = a.job;
+      }
     }
   }
 }
@@ -414,37 +421,15 @@ constexpr int qsieve_lds_bytes() {
   return (PT.off[NL] + 90) * 8 + WAVES * (LV::N - 1) * 128 * (int)sizeof(QEntry);
 }
 
-/* VC: keep the level-0 constants in VGPRs */
-template <int NL, int WAVES, class LV, bool VC>
-__global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
-sieve_queue_kernel(SieveArgs a) {
-  static_assert(qsieve_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
-  static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
-  constexpr int LMAIN = PT.off[NL];
-  constexpr int LFILT = 90;
-  using QS = QSieve<NL, LV>;
-  __shared__ uint64_t tab[LMAIN + LFILT];
-  __shared__ QEntry queues[WAVES][QS::NQ][QS::QCAP];
-  for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) tab[i] = a.tab[i];
-  for (int i = threadIdx.x; i < LFILT; i += blockDim.x) tab[LMAIN + i] = a.tab[NOK + i];
-  __syncthreads();
-
-  const int wave = threadIdx.x >> 6;
-  QS qs;
-  qs.lane = threadIdx.x & 63;
-  qs.q = queues[wave];
-  for (int i = 0; i < QS::NQ; i++) qs.qn[i] = 0;
-  qs.ldsb = (const char *)tab;
-  qs.gtb = (const char *)a.tab;
-  qs.a = &a;
-  VConsts<0, LV::B[1]> vc;
-  if constexpr (VC) vc.init();
-
+/* The positions of n53 blocks [blk0, blk1) taken by this thread at tid,
+   tid+stride, ...: level 0 on every lane, queues for the rest, then flush. */
+template <int NL, class LV, bool VC>
+__device__ __forceinline__ void qsieve_blocks(QSieve<NL, LV> &qs, const VConsts<0, LV::B[1]> &vc,
+                                              const SieveArgs &a, int64_t tid, int64_t stride) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  int64_t b = a.blk0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t b = a.blk0 + tid;
   bool has = b < a.blk1;
   double xd = has ? (double)n53_of_block(a.kp, a.n43, b) + TWO52 : TWO52;
   int i59 = 0;
@@ -472,6 +457,238 @@ sieve_queue_kernel(SieveArgs a) {
   qs.template flush<0>();
 }
 
+template <int NL, int WAVES, class LV>
+struct QSieveLds {
+  uint64_t tab[PT.off[NL] + 90];
+  QEntry queues[WAVES][LV::N - 1][128];
+};
+
+/* One launch per (K, shift): the plain path.  VC: level-0 constants in VGPRs. */
+template <int NL, int WAVES, class LV, bool VC>
+static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+sieve_queue_kernel(SieveArgs a) {
+  static_assert(qsieve_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
+  constexpr int LMAIN = PT.off[NL];
+  __shared__ QSieveLds<NL, WAVES, LV> L;
+  for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) L.tab[i] = a.tab[i];
+  for (int i = threadIdx.x; i < 90; i += blockDim.x) L.tab[LMAIN + i] = a.tab[NOK + i];
+  __syncthreads();
+
+  QSieve<NL, LV> qs;
+  qs.lane = threadIdx.x & 63;
+  qs.q = L.queues[threadIdx.x >> 6];
+  for (int i = 0; i < QSieve<NL, LV>::NQ; i++) qs.qn[i] = 0;
+  qs.ldsb = (const char *)L.tab;
+  qs.gtb = (const char *)a.tab;
+  qs.a = &a;
+  VConsts<0, LV::B[1]> vc;
+  if constexpr (VC) vc.init();
+  qsieve_blocks<NL, LV, VC>(qs, vc, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                            (int64_t)gridDim.x * blockDim.x);
+}
+
+/* ---- batched persistent sieve ------------------------------------------ */
+
+struct BatchJobDev {
+  KParams kp;        /* K, SHIFT (the unit's), STEP, n0, S31..S59 */
+  int32_t it0;       /* first shift iteration searched */
+  int32_t range_id;
+};
+
+struct RoundState {
+  int32_t base;      /* first job of the current round */
+  int32_t njobs;     /* jobs in the whole batch */
+  int32_t per_round; /* jobs per round */
+  int32_t pad;
+};
+
+struct PersistArgs {
+  const BatchJobDev *__restrict__ jobs;
+  const RoundState *__restrict__ rs;
+  const int64_t *__restrict__ n43;   /* per_round x 10840 */
+  const uint64_t *__restrict__ tab;  /* per_round x 10 x NTAB */
+  int64_t *__restrict__ surv;
+  int32_t *__restrict__ surv_job;
+  unsigned long long *__restrict__ nsurv;
+  int64_t cap;
+  unsigned int *__restrict__ work;   /* work-stealing item counter */
+  int32_t chunk;                     /* n43 starts per item */
+  int32_t nchunk;
+};
+
+/* Work items (job, shift, n43 chunk) of one round, taken from a global atomic
+   counter by whichever workgroup is free: long and short ranges, mixed SHIFTs
+   and resumed units share one launch.  A workgroup restages its LDS tables
+   only when the (job, shift) of its next item differs. */
+template <int NL, int WAVES, class LV>
+static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
+sieve_persistent_kernel(PersistArgs p) {
+  static_assert(qsieve_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  constexpr int LMAIN = PT.off[NL];
+  __shared__ QSieveLds<NL, WAVES, LV> L;
+  __shared__ unsigned int s_item;
+  const RoundState rs = *p.rs;
+  const int njobs = min(rs.per_round, rs.njobs - rs.base);
+  const unsigned int nitems = (unsigned int)(njobs > 0 ? njobs : 0) * AP26_NSHIFT * p.nchunk;
+  int cur = -1;
+  VConsts<0, LV::B[1]> vc;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
+    __syncthreads();
+    const unsigned int item = s_item;
+    if (item >= nitems) break;
+    const int jl = (int)(item / (AP26_NSHIFT * p.nchunk));
+    const int rem = (int)(item - (unsigned int)jl * AP26_NSHIFT * p.nchunk);
+    const int it = rem / p.nchunk;
+    const int chunk = rem - it * p.nchunk;
+    const BatchJobDev &job = p.jobs[rs.base + jl];
+    if (it < job.it0) continue;
+    const uint64_t *tab = p.tab + ((size_t)jl * AP26_NSHIFT + it) * NTAB;
+    const int key = jl * AP26_NSHIFT + it;
+    if (key != cur) {
+      for (int i = threadIdx.x; i < LMAIN; i += blockDim.x) L.tab[i] = tab[i];
+      for (int i = threadIdx.x; i < 90; i += blockDim.x) L.tab[LMAIN + i] = tab[NOK + i];
+      __syncthreads();
+      cur = key;
+    }
+    SieveArgs a;
+    a.kp = job.kp;
+    a.n43 = p.n43 + (size_t)jl * AP26_NN43;
+    a.tab = tab;
+    a.shift0 = job.kp.SHIFT + 64 * it;
+    a.nslot = 1;
+    const int j0 = chunk * p.chunk;
+    const int j1 = min(AP26_NN43, j0 + p.chunk);
+    a.blk0 = (int64_t)j0 * AP26_N53_PER_N43;
+    a.blk1 = (int64_t)j1 * AP26_N53_PER_N43;
+    a.surv = p.surv;
+    a.nsurv = p.nsurv;
+    a.cap = p.cap;
+    a.surv_job = p.surv_job;
+    a.job = jl;
+    QSieve<NL, LV> qs;
+    qs.lane = threadIdx.x & 63;
+    qs.q = L.queues[threadIdx.x >> 6];
+    for (int i = 0; i < QSieve<NL, LV>::NQ; i++) qs.qn[i] = 0;
+    qs.ldsb = (const char *)L.tab;
+    qs.gtb = (const char *)tab;
+    qs.a = &a;
+    qsieve_blocks<NL, LV, false>(qs, vc, a, threadIdx.x, blockDim.x);
+  }
+}
+
+/* n43 starts of every job in the round, from the constant list of guarded
+   (i31, i37, i41) triples (src/opencl/AP26.h:125-136). */
+static __global__ void n43_kernel(const BatchJobDev *__restrict__ jobs, const RoundState *__restrict__ rsp,
+                           const uint32_t *__restrict__ triples, int64_t *__restrict__ n43) {
+  const RoundState rs = *rsp;
+  const int njobs = min(rs.per_round, rs.njobs - rs.base);
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jl = idx / AP26_NN43;
+  if (jl >= njobs) return;
+  const int j = idx - jl * AP26_NN43;
+  const uint32_t t = triples[j >> 3];
+  const int i31 = t & 0xff, i37 = (t >> 8) & 0xff, i41 = (t >> 16) & 0xff;
+  const int i3 = (j >> 2) & 1, i5 = j & 3;
+  const KParams &kp = jobs[rs.base + jl].kp;
+  int64_t v = kp.n0 + i3 * AP26_S3 + i5 * AP26_S5 + i31 * kp.S31 + i37 * kp.S37 + i41 * kp.S41;
+  int64_t q = (int64_t)((double)v * (1.0 / (double)AP26_MOD));
+  v -= q * AP26_MOD;
+  if (v < 0) v += AP26_MOD;
+  if (v >= AP26_MOD) v -= AP26_MOD;
+  n43[(size_t)jl * AP26_NN43 + j] = v;
+}
+
+/* Tables of every (job, shift) of the round; grid z = job*10 + shift. */
+static __global__ void batch_tables_kernel(const BatchJobDev *__restrict__ jobs, const RoundState *__restrict__ rsp,
+                                    uint64_t *__restrict__ tab) {
+  const RoundState rs = *rsp;
+  const int njobs = min(rs.per_round, rs.njobs - rs.base);
+  const int jl = blockIdx.z / AP26_NSHIFT;
+  const int it = blockIdx.z - jl * AP26_NSHIFT;
+  if (jl >= njobs) return;
+  const int ti = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= PT.p[ti]) return;
+  const KParams &kp = jobs[rs.base + jl].kp;
+  tab[((size_t)jl * AP26_NSHIFT + it) * NTAB + PT.off[ti] + c] = table_word(kp, ti, c, kp.SHIFT + 64 * it);
+}
+
+struct BatchWalkArgs {
+  const BatchJobDev *__restrict__ jobs;
+  const RoundState *__restrict__ rs;
+  const int64_t *__restrict__ surv;
+  const int32_t *__restrict__ surv_job;
+  const unsigned long long *__restrict__ nsurv;
+  int64_t cap;
+  DevSol *__restrict__ sols;
+  unsigned int *__restrict__ nsol;
+  int32_t sol_cap;
+  unsigned long long *__restrict__ ntests;
+  unsigned long long *__restrict__ hash_tot; /* per range: exact sum of first%1000 + k */
+};
+
+__device__ __forceinline__ int walk_one(int64_t c, int64_t STEP, int64_t *first, unsigned long long &tests) {
+  int k = 0;
+  int64_t m = c + 5 * STEP;
+  while (true) {
+    tests++;
+    if (!sprp2((uint64_t)m)) break;
+    k++;
+    m += STEP;
+  }
+  if (k >= 10) {
+    m = c + 4 * STEP;
+    while (m > 0) {
+      tests++;
+      if (!sprp2((uint64_t)m)) break;
+      k++;
+      m -= STEP;
+    }
+  }
+  *first = m + STEP;
+  return k;
+}
+
+static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a) {
+  const RoundState rs = *a.rs;
+  unsigned long long n = *a.nsurv;
+  if (n > (unsigned long long)a.cap) n = (unsigned long long)a.cap;
+  unsigned long long tests = 0;
+  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (unsigned long long)gridDim.x * blockDim.x) {
+    const int64_t c = a.surv[i];
+    const BatchJobDev &job = a.jobs[rs.base + a.surv_job[i]];
+    int64_t first;
+    const int k = walk_one(c, job.kp.STEP, &first, tests);
+    if (k >= 10) {
+      atomicAdd(&a.hash_tot[job.range_id], (unsigned long long)(first % 1000 + k));
+      const unsigned int idx = atomicAdd(a.nsol, 1u);
+      if (idx < (unsigned int)a.sol_cap) {
+        DevSol s;
+        s.k = k;
+        s.K = job.kp.K;
+        s.first = first;
+        s.range_id = job.range_id;
+        s.shift_iter = (int32_t)((c / AP26_MOD - job.kp.SHIFT) / 64);
+        a.sols[idx] = s;
+      }
+    }
+  }
+  if (tests) atomicAdd(a.ntests, tests);
+}
+
+/* End of round: clear the per-round counters, advance to the next round. */
+static __global__ void round_advance_kernel(RoundState *rs, unsigned long long *nsurv, unsigned int *work) {
+  if (threadIdx.x == 0) {
+    rs->base += rs->per_round;
+    *nsurv = 0;
+    *work = 0;
+  }
+}
+
 /* ---- walk -------------------------------------------------------------- */
 /* For each survivor n: count base-2 SPRPs forward from n+5*STEP; only if the
  * count reaches 10, extend backward from n+4*STEP while the term is positive;
@@ -489,7 +706,7 @@ struct WalkArgs {
   unsigned long long *__restrict__ ntests;
 };
 
-__global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
+static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   unsigned long long n = *a.nsurv;
   if (n > (unsigned long long)a.cap) n = (unsigned long long)a.cap;
   unsigned long long tests = 0;
@@ -530,7 +747,7 @@ __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
 }
 
 /* ---- batched SPRP ------------------------------------------------------ */
-__global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restrict__ n, uint8_t *__restrict__ out,
+static __global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restrict__ n, uint8_t *__restrict__ out,
                                                    uint64_t count) {
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
        i += (uint64_t)gridDim.x * blockDim.x)
@@ -539,7 +756,7 @@ __global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restrict__ n
 
 /* Clear counters with a kernel (not a memset node), so the sequence can be
    captured into a graph. */
-__global__ void zero_kernel(unsigned long long *p, int n) {
+static __global__ void zero_kernel(unsigned long long *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
