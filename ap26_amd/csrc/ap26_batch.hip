@@ -11,6 +11,8 @@
 
 #include <chrono>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <vector>
 
 #include "ap26_ctx.h"
@@ -68,8 +70,7 @@ struct BatchState {
   int graph_njobs = -1;      /* graph is valid for this per_round/chunk setup */
 };
 
-void batch_free(ap26_ctx *ctx) {
-  BatchState *b = ctx->batch;
+static void batch_state_free(BatchState *b) {
   if (!b) return;
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   (void)hipFree(b->d_jobs);
@@ -83,18 +84,21 @@ void batch_free(ap26_ctx *ctx) {
   (void)hipFree(b->d_sols);
   (void)hipFree(b->d_hash);
   delete b;
+}
+
+/* per_round = K held by one round (device memory ~0.75 GB per K) */
+void batch_free(ap26_ctx *ctx) {
+  batch_state_free(ctx->batch);
   ctx->batch = nullptr;
 }
 
-static int batch_init(ap26_ctx *ctx) {
-  if (ctx->batch) return AP26_OK;
+static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
+  *out = nullptr;
   BatchState *b = new (std::nothrow) BatchState();
   if (!b) return AP26_E_NOMEM;
-  ctx->batch = b;
-  const char *e = getenv("AP26_BATCH_ROUND");
-  b->per_round = e ? atoi(e) : 16;
-  if (b->per_round < 1 || b->per_round > 64) b->per_round = 16;
-  e = getenv("AP26_BATCH_CHUNK");
+  int rc = AP26_OK;
+  b->per_round = per_round;
+  const char *e = getenv("AP26_BATCH_CHUNK");
   b->chunk = e ? atoi(e) : 85;
   if (b->chunk < 1 || b->chunk > AP26_NN43) b->chunk = 85;
   b->nchunk = (AP26_NN43 + b->chunk - 1) / b->chunk;
@@ -112,22 +116,36 @@ static int batch_init(ap26_ctx *ctx) {
         trip.push_back((uint32_t)i31 | ((uint32_t)i37 << 8) | ((uint32_t)i41 << 16));
       }
     }
-  if (trip.size() * 8 != AP26_NN43) return AP26_E_INTERNAL;
-  if (hipMalloc(&b->d_rs, sizeof(RoundState)) != hipSuccess ||
-      hipMalloc(&b->d_triples, trip.size() * 4) != hipSuccess ||
-      hipMalloc(&b->d_n43, sizeof(int64_t) * AP26_NN43 * b->per_round) != hipSuccess ||
-      hipMalloc(&b->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT * b->per_round) != hipSuccess ||
-      hipMalloc(&b->d_surv, sizeof(int64_t) * b->surv_cap) != hipSuccess ||
-      hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess ||
-      hipMalloc(&b->d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
-      hipMalloc(&b->d_sols, sizeof(DevSol) * BATCH_SOL_CAP) != hipSuccess)
-    return AP26_E_NOMEM;
-  HIPCHK(hipMemcpy(b->d_triples, trip.data(), trip.size() * 4, hipMemcpyHostToDevice));
+  if (trip.size() * 8 != AP26_NN43) rc = AP26_E_INTERNAL;
+  if (!rc && (hipMalloc(&b->d_rs, sizeof(RoundState)) != hipSuccess ||
+              hipMalloc(&b->d_triples, trip.size() * 4) != hipSuccess ||
+              hipMalloc(&b->d_n43, sizeof(int64_t) * AP26_NN43 * b->per_round) != hipSuccess ||
+              hipMalloc(&b->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT * b->per_round) != hipSuccess ||
+              hipMalloc(&b->d_surv, sizeof(int64_t) * b->surv_cap) != hipSuccess ||
+              hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess ||
+              hipMalloc(&b->d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
+              hipMalloc(&b->d_sols, sizeof(DevSol) * BATCH_SOL_CAP) != hipSuccess))
+    rc = AP26_E_NOMEM;
+  if (!rc && hipMemcpy(b->d_triples, trip.data(), trip.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+    rc = AP26_E_DEVICE;
+  if (rc) {
+    batch_state_free(b);
+    return rc;
+  }
+  (void)ctx;
+  *out = b;
   return AP26_OK;
 }
 
-static int ensure_capacity(ap26_ctx *ctx, int njobs, int nranges) {
-  BatchState *b = ctx->batch;
+static int batch_init(ap26_ctx *ctx) {
+  if (ctx->batch) return AP26_OK;
+  const char *e = getenv("AP26_BATCH_ROUND");
+  int per_round = e ? atoi(e) : 16;
+  if (per_round < 1 || per_round > 64) per_round = 16;
+  return batch_state_new(ctx, per_round, &ctx->batch);
+}
+
+static int ensure_capacity(BatchState *b, int njobs, int nranges) {
   if (njobs > b->max_jobs) {
     (void)hipFree(b->d_jobs);
     b->d_jobs = nullptr;
@@ -154,8 +172,7 @@ static int ensure_capacity(ap26_ctx *ctx, int njobs, int nranges) {
 }
 
 /* enqueue one round on ctx->stream (captured into the graph) */
-static int enqueue_round(ap26_ctx *ctx) {
-  BatchState *b = ctx->batch;
+static int enqueue_round(ap26_ctx *ctx, BatchState *b) {
   const int R = b->per_round;
   hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, b->d_jobs,
                      b->d_rs, b->d_triples, b->d_n43);
@@ -192,6 +209,43 @@ static int enqueue_round(ap26_ctx *ctx) {
   return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
 }
 
+static int ensure_graph(ap26_ctx *ctx, BatchState *b) {
+  if (b->graph) return AP26_OK;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_round(ctx, b);
+  hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+  if (rc || ce != hipSuccess) return AP26_E_DEVICE;
+  hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  return ie == hipSuccess ? AP26_OK : AP26_E_DEVICE;
+}
+
+static BatchJobDev make_job(int32_t K, int32_t shift, int32_t it0, int32_t range_id) {
+  BatchJobDev j;
+  HostK hk;
+  host_kconsts(K, &hk);
+  j.kp.STEP = hk.STEP;
+  j.kp.n0 = hk.n0;
+  j.kp.S31 = hk.S[0];
+  j.kp.S37 = hk.S[1];
+  j.kp.S41 = hk.S[2];
+  j.kp.S43 = hk.S[3];
+  j.kp.S47 = hk.S[4];
+  j.kp.S53 = hk.S[5];
+  j.kp.S59 = hk.S[6];
+  j.kp.K = K;
+  j.kp.SHIFT = shift;
+  j.it0 = it0;
+  j.range_id = range_id;
+  return j;
+}
+
+static bool unit_ok(const ap26_range &u) {
+  return !(u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || u.shift < 0 || u.shift > (1 << 20) ||
+           u.iter_start < 0 || u.iter_start > AP26_NSHIFT);
+}
+
 __global__ void batch_reset_kernel(RoundState *rs, int njobs, int per_round, unsigned long long *cnt,
                                    unsigned long long *hash, int nranges) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -217,31 +271,13 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   std::vector<BatchJobDev> jobs;
   for (int i = 0; i < nr; i++) {
     const ap26_range &u = r[i];
-    if (u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || u.shift < 0 || u.shift > (1 << 20) ||
-        u.iter_start < 0 || u.iter_start > AP26_NSHIFT)
-      return AP26_E_RANGE;
+    if (!unit_ok(u)) return AP26_E_RANGE;
     for (int64_t K = u.kmin; K <= u.kmax; K++) {
       if (!host_will_search((int32_t)K)) {
         st.k_skipped++;
         continue;
       }
-      BatchJobDev j;
-      HostK hk;
-      host_kconsts((int32_t)K, &hk);
-      j.kp.STEP = hk.STEP;
-      j.kp.n0 = hk.n0;
-      j.kp.S31 = hk.S[0];
-      j.kp.S37 = hk.S[1];
-      j.kp.S41 = hk.S[2];
-      j.kp.S43 = hk.S[3];
-      j.kp.S47 = hk.S[4];
-      j.kp.S53 = hk.S[5];
-      j.kp.S59 = hk.S[6];
-      j.kp.K = (int32_t)K;
-      j.kp.SHIFT = u.shift;
-      j.it0 = (K == u.kmin) ? u.iter_start : 0;
-      j.range_id = i;
-      jobs.push_back(j);
+      jobs.push_back(make_job((int32_t)K, u.shift, (K == u.kmin) ? u.iter_start : 0, i));
     }
   }
   st.k_searched = (int64_t)jobs.size();
@@ -254,7 +290,7 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   int rc = batch_init(ctx);
   if (rc) return rc;
   BatchState *b = ctx->batch;
-  rc = ensure_capacity(ctx, (int)jobs.size(), nr);
+  rc = ensure_capacity(b, (int)jobs.size(), nr);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(BatchJobDev) * jobs.size(), hipMemcpyHostToDevice,
                         ctx->stream));
@@ -262,16 +298,8 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs,
                      (int)jobs.size(), b->per_round, b->d_cnt, b->d_hash, nr);
   HIPCHK(hipGetLastError());
-  if (!b->graph) {
-    hipGraph_t g;
-    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_round(ctx);
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-    if (rc || ce != hipSuccess) return AP26_E_DEVICE;
-    hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) return AP26_E_DEVICE;
-  }
+  rc = ensure_graph(ctx, b);
+  if (rc) return rc;
   const int rounds = (int)((jobs.size() + b->per_round - 1) / b->per_round);
   HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
   for (int i = 0; i < rounds; i++) HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
@@ -308,4 +336,245 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
+/* ---- serving ------------------------------------------------------------ */
+
+struct SUnit {
+  ap26_range r{};
+  std::vector<int32_t> ks; /* searched K */
+  size_t next = 0;         /* next K to schedule */
+  int outstanding = 0;     /* K in the running round */
+  int slot = -1;
+  double t_submit = 0, t_done = 0;
+  std::vector<ap26_sol> sols;
+  int64_t hash = 0;
+  bool done = false;
+};
+
+struct ap26_server {
+  ap26_ctx *ctx = nullptr;
+  BatchState *b = nullptr;
+  int max_units = 0;
+  int R = 0;
+  std::map<int64_t, SUnit> units; /* admitted, not yet collected */
+  std::deque<int64_t> active;     /* admission order, units with K left */
+  std::vector<int> free_slots;
+  int64_t next_ticket = 1;
+  int64_t units_done = 0, k_done = 0, rounds = 0;
+  size_t rr = 0;                  /* round-robin cursor */
+  BatchJobDev *h_jobs = nullptr;  /* pinned staging */
+  RoundState *h_rs = nullptr;
+  unsigned long long *h_cnt = nullptr;
+  unsigned long long *h_hash = nullptr;
+};
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+__global__ void server_zero_kernel(unsigned long long *cnt) {
+  if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
+}
+
+__global__ void server_zero_slot_kernel(unsigned long long *hash, int slot) {
+  if (threadIdx.x == 0) hash[slot] = 0;
+}
+
+static void server_free(ap26_server *s) {
+  if (!s) return;
+  batch_state_free(s->b);
+  if (s->h_jobs) (void)hipHostFree(s->h_jobs);
+  if (s->h_rs) (void)hipHostFree(s->h_rs);
+  if (s->h_cnt) (void)hipHostFree(s->h_cnt);
+  if (s->h_hash) (void)hipHostFree(s->h_hash);
+  delete s;
+}
+
+extern "C" int ap26_server_open(ap26_ctx *ctx, int32_t max_units, int64_t mem_budget_bytes, ap26_server **out) {
+  if (!ctx || !out || max_units < 1 || max_units > (1 << 16)) return AP26_E_RANGE;
+  *out = nullptr;
+  HIPCHK(hipSetDevice(ctx->device));
+  ap26_server *s = new (std::nothrow) ap26_server();
+  if (!s) return AP26_E_NOMEM;
+  s->ctx = ctx;
+  s->max_units = max_units;
+  int64_t budget = mem_budget_bytes > 0 ? mem_budget_bytes : (int64_t)(ctx->global_mem / 8);
+  const int64_t per_k = (int64_t)NTAB * AP26_NSHIFT * 8 + (int64_t)AP26_NN43 * 8 + (INT64_C(60) << 20) * 12;
+  int R = (int)(budget / per_k);
+  if (R < 1) {
+    delete s;
+    return AP26_E_NOMEM; /* not even one K fits the budget */
+  }
+  if (R > 64) R = 64;
+  s->R = R;
+  int rc = batch_state_new(ctx, R, &s->b);
+  if (!rc) rc = ensure_capacity(s->b, R, max_units);
+  if (!rc && (hipHostMalloc((void **)&s->h_jobs, sizeof(BatchJobDev) * R, hipHostMallocDefault) != hipSuccess ||
+              hipHostMalloc((void **)&s->h_rs, sizeof(RoundState), hipHostMallocDefault) != hipSuccess ||
+              hipHostMalloc((void **)&s->h_cnt, sizeof(unsigned long long) * 8, hipHostMallocDefault) != hipSuccess ||
+              hipHostMalloc((void **)&s->h_hash, sizeof(unsigned long long) * max_units, hipHostMallocDefault) !=
+                  hipSuccess))
+    rc = AP26_E_NOMEM;
+  if (!rc) {
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((max_units + 255) / 256), dim3(256), 0, ctx->stream, s->b->d_rs, 0,
+                       R, s->b->d_cnt, s->b->d_hash, max_units);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) rc = AP26_E_DEVICE;
+  }
+  if (!rc) rc = ensure_graph(ctx, s->b);
+  if (rc) {
+    server_free(s);
+    return rc;
+  }
+  for (int i = max_units - 1; i >= 0; i--) s->free_slots.push_back(i);
+  *out = s;
+  return AP26_OK;
+}
+
+extern "C" int ap26_server_close(ap26_server *s) {
+  if (!s) return AP26_OK;
+  ap26_ctx *ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  server_free(s);
+  return AP26_OK;
+}
+
+extern "C" int ap26_server_submit(ap26_server *s, const ap26_range *unit, int64_t *ticket) {
+  if (!s || !unit || !ticket) return AP26_E_RANGE;
+  if (!unit_ok(*unit)) return AP26_E_RANGE;
+  if (s->free_slots.empty()) return AP26_E_BUSY;
+  SUnit u;
+  u.r = *unit;
+  for (int64_t K = unit->kmin; K <= unit->kmax; K++)
+    if (host_will_search((int32_t)K)) u.ks.push_back((int32_t)K);
+  u.slot = s->free_slots.back();
+  s->free_slots.pop_back();
+  u.t_submit = now_ms();
+  const int64_t t = s->next_ticket++;
+  if (u.ks.empty()) { /* nothing to search: done at once */
+    u.done = true;
+    u.t_done = u.t_submit;
+    s->free_slots.push_back(u.slot);
+    u.slot = -1;
+    s->units_done++;
+  } else {
+    s->active.push_back(t);
+  }
+  s->units.emplace(t, std::move(u));
+  *ticket = t;
+  return AP26_OK;
+}
+
+extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
+  if (!s) return AP26_E_RANGE;
+  ap26_ctx *ctx = s->ctx;
+  BatchState *b = s->b;
+  if (completed) *completed = 0;
+  if (s->active.empty()) return AP26_OK;
+  HIPCHK(hipSetDevice(ctx->device));
+  /* one K from each active unit in turn until the round is full */
+  std::vector<int64_t> owner;
+  int n = 0;
+  bool progress = true;
+  while (n < s->R && progress) {
+    progress = false;
+    for (size_t c = 0; c < s->active.size() && n < s->R; c++) {
+      const size_t idx = (s->rr + c) % s->active.size();
+      SUnit &u = s->units[s->active[idx]];
+      if (u.next >= u.ks.size()) continue;
+      const int32_t K = u.ks[u.next];
+      const int it0 = (u.next == 0 && K == u.r.kmin) ? u.r.iter_start : 0;
+      u.next++;
+      u.outstanding++;
+      s->h_jobs[n++] = make_job(K, u.r.shift, it0, u.slot);
+      owner.push_back(s->active[idx]);
+      progress = true;
+    }
+  }
+  s->rr = s->active.empty() ? 0 : (s->rr + 1) % s->active.size();
+  if (n == 0) return AP26_OK;
+  s->h_rs->base = 0;
+  s->h_rs->njobs = n;
+  s->h_rs->per_round = s->R;
+  s->h_rs->pad = 0;
+  HIPCHK(hipMemcpyAsync(b->d_jobs, s->h_jobs, sizeof(BatchJobDev) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  s->rounds++;
+  s->k_done += n;
+  const unsigned int nsol = (unsigned int)(s->h_cnt[2] & 0xffffffffu);
+  if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
+  std::vector<DevSol> hs(nsol);
+  if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+  std::map<int, int64_t> slot_owner;
+  for (int64_t t : owner) slot_owner[s->units[t].slot] = t;
+  for (const DevSol &d : hs) {
+    auto it = slot_owner.find(d.range_id);
+    if (it == slot_owner.end()) return AP26_E_INTERNAL;
+    ap26_sol o{d.k, d.K, d.first, -1, d.shift_iter};
+    s->units[it->second].sols.push_back(o);
+  }
+  std::vector<int> cleared;
+  const double t = now_ms();
+  for (int64_t tk : owner) {
+    SUnit &u = s->units[tk];
+    u.outstanding--;
+    if (u.outstanding == 0 && u.next >= u.ks.size() && !u.done) {
+      u.done = true;
+      u.t_done = t;
+      u.hash = (int64_t)s->h_hash[u.slot];
+      cleared.push_back(u.slot);
+      s->free_slots.push_back(u.slot);
+      u.slot = -1;
+      s->units_done++;
+      if (completed) (*completed)++;
+      for (size_t i = 0; i < s->active.size(); i++)
+        if (s->active[i] == tk) {
+          s->active.erase(s->active.begin() + (long)i);
+          break;
+        }
+    }
+  }
+  if (s->rr >= s->active.size()) s->rr = 0;
+  /* clear the round counters (sols, tests) and the finished units' totals */
+  for (int slot : cleared)
+    hipLaunchKernelGGL(server_zero_slot_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_hash, slot);
+  hipLaunchKernelGGL(server_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_cnt);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  return AP26_OK;
+}
+
+extern "C" int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out, int32_t cap, int32_t *n_out,
+                                  int64_t *hash_total, double *latency_ms, int32_t *done) {
+  if (!s || !done) return AP26_E_RANGE;
+  auto it = s->units.find(ticket);
+  if (it == s->units.end()) return AP26_E_RANGE;
+  SUnit &u = it->second;
+  *done = u.done ? 1 : 0;
+  if (!u.done) return AP26_OK;
+  const int n = (int)u.sols.size();
+  if (n_out) *n_out = n;
+  if (hash_total) *hash_total = u.hash;
+  if (latency_ms) *latency_ms = u.t_done - u.t_submit;
+  if (n > cap) return AP26_E_OVERFLOW; /* kept: call again with a larger buffer */
+  for (int i = 0; i < n; i++) out[i] = u.sols[i];
+  s->units.erase(it);
+  return AP26_OK;
+}
+
+extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int64_t *rounds,
+                                 int32_t *k_per_round, int32_t *active) {
+  if (!s) return AP26_E_RANGE;
+  if (units_done) *units_done = s->units_done;
+  if (k_done) *k_done = s->k_done;
+  if (rounds) *rounds = s->rounds;
+  if (k_per_round) *k_per_round = s->R;
+  if (active) *active = (int32_t)s->active.size();
+  return AP26_OK;
 }

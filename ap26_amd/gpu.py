@@ -79,6 +79,16 @@ SIGNATURES = {
                                           ctypes.POINTER(ctypes.c_double)]),
     "ap26_search_batch": (ctypes.c_int, [_P, _P, _I32, _P, _I32, ctypes.POINTER(_I32), _P,
                                          ctypes.POINTER(Stats)]),
+    "ap26_server_open": (ctypes.c_int, [_P, _I32, _I64, ctypes.POINTER(_P)]),
+    "ap26_server_close": (ctypes.c_int, [_P]),
+    "ap26_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Range), ctypes.POINTER(_I64)]),
+    "ap26_server_pump": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+    "ap26_server_result": (ctypes.c_int, [_P, _I64, _P, _I32, ctypes.POINTER(_I32),
+                                          ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(_I32)]),
+    "ap26_server_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                         ctypes.POINTER(_I64), ctypes.POINTER(_I32),
+                                         ctypes.POINTER(_I32)]),
     "ap26_dev_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "ap26_dev_free": (ctypes.c_int, [_P, _P]),
     "ap26_copy_h2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
@@ -232,3 +242,63 @@ class Context:
 
     def sync(self):
         _check(self._lib.ap26_sync(self.h), "ap26_sync")
+
+
+class Server:
+    """Continuous-batching server on a Context (ap26_server_* in the C ABI)."""
+
+    def __init__(self, ctx, max_units=256, mem_budget_bytes=0):
+        self._lib = ctx._lib
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(self._lib.ap26_server_open(ctx.h, max_units, mem_budget_bytes, ctypes.byref(h)),
+               "ap26_server_open")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._lib.ap26_server_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, kmin, kmax, shift, iter_start=0):
+        """Returns a ticket, or None when the server is full (AP26_E_BUSY)."""
+        r = Range(kmin, kmax, shift, iter_start)
+        t = ctypes.c_int64()
+        rc = self._lib.ap26_server_submit(self.h, ctypes.byref(r), ctypes.byref(t))
+        if rc == AP26_E_BUSY:
+            return None
+        _check(rc, "ap26_server_submit")
+        return t.value
+
+    def pump(self):
+        n = ctypes.c_int32()
+        _check(self._lib.ap26_server_pump(self.h, ctypes.byref(n)), "ap26_server_pump")
+        return n.value
+
+    def result(self, ticket, cap=1 << 16):
+        """None while running, else (reports, hash_total, latency_ms)."""
+        out = np.zeros(cap, dtype=SOL_DTYPE)
+        n = ctypes.c_int32()
+        h = ctypes.c_int64()
+        lat = ctypes.c_double()
+        done = ctypes.c_int32()
+        rc = self._lib.ap26_server_result(self.h, ticket, _ptr(out), cap, ctypes.byref(n),
+                                          ctypes.byref(h), ctypes.byref(lat), ctypes.byref(done))
+        _check(rc, "ap26_server_result")
+        if not done.value:
+            return None
+        return out[:n.value].copy(), h.value, lat.value
+
+    def stats(self):
+        ud, kd, rd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        kpr, act = ctypes.c_int32(), ctypes.c_int32()
+        _check(self._lib.ap26_server_stats(self.h, ctypes.byref(ud), ctypes.byref(kd), ctypes.byref(rd),
+                                           ctypes.byref(kpr), ctypes.byref(act)), "ap26_server_stats")
+        return {"units_done": ud.value, "k_done": kd.value, "rounds": rd.value,
+                "k_per_round": kpr.value, "active": act.value}

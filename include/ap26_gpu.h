@@ -124,6 +124,32 @@ int ap26_prp_batch_dev(ap26_ctx *ctx, const void *d_n, void *d_out, size_t count
 int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out,
                       int32_t cap, int32_t *n_out, int64_t *hash_totals, ap26_stats *stats);
 
+/* ---- serving: continuous batching of work units ------------------------
+   Units are admitted while unit slots and device memory allow; every round
+   takes the next K of each active unit in turn (round-robin), so units that
+   arrive while others run join the next round and share the device with
+   them.  Reports and per-unit hash totals are kept per unit (totals on the
+   device) until collected.  Single host thread per server. */
+typedef struct ap26_server ap26_server;
+
+/* max_units: admitted units (active + waiting); mem_budget_bytes: device
+   memory the server may use for in-flight K (0 = 1/8 of the GPU); it sets how
+   many K a round holds. */
+int ap26_server_open(ap26_ctx *ctx, int32_t max_units, int64_t mem_budget_bytes, ap26_server **out);
+int ap26_server_close(ap26_server *s);
+/* Queue a unit; AP26_E_BUSY if max_units are already admitted. */
+int ap26_server_submit(ap26_server *s, const ap26_range *unit, int64_t *ticket);
+/* Launch the next round if work is waiting and harvest the finished one;
+   *completed = units finished by this call.  Blocks for one round. */
+int ap26_server_pump(ap26_server *s, int32_t *completed);
+/* Result of a finished unit (done = 0 while it runs): its raw reports (range_id
+   = -1), the exact hash total, and its latency from submit to completion. */
+int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out, int32_t cap, int32_t *n_out,
+                       int64_t *hash_total, double *latency_ms, int32_t *done);
+/* Counters: units admitted/finished, K searched, rounds, K per round. */
+int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int64_t *rounds,
+                      int32_t *k_per_round, int32_t *active);
+
 /* Device buffer helpers (so callers need no HIP/torch to stage inputs). */
 int ap26_dev_alloc(ap26_ctx *ctx, size_t bytes, void **dptr);
 int ap26_dev_free(ap26_ctx *ctx, void *dptr);

@@ -1,0 +1,122 @@
+"""Batched engine (ap26_search_batch) and serving loop (ap26_server_*):
+every range / unit gets the same reports and hash total as the plain per-K
+path and the reference golden files, alone and in a mix."""
+import os
+
+import numpy as np
+import pytest
+
+from ap26_amd import gpu
+from ap26_amd import host as H
+from conftest import GOLDEN, read_sol
+
+pytestmark = pytest.mark.gpu
+
+
+def will_search(K):
+    return all(K % p for p in (29, 31, 37, 41, 43, 47, 53, 59))
+
+
+def plain_reports(ctx, kmin, kmax, shift, it=0):
+    out = []
+    for K in range(kmin, kmax + 1):
+        if will_search(K):
+            s, _ = ctx.search_k(K, shift, it if K == kmin else 0)
+            out += [(int(x["k"]), int(x["K"]), int(x["first"])) for x in s]
+    return sorted(out)
+
+
+def key(sols):
+    return sorted((int(x["k"]), int(x["K"]), int(x["first"])) for x in sols)
+
+
+RANGES = [(366384, 366384, 0, 0), (875858, 875860, 640, 0), (44121555, 44121555, 0, 0),
+          (875861, 875861, 640, 4), (99999989, 99999989, 1280, 0), (976061, 976062, 0, 0),
+          (29, 29, 0, 0), (1000003, 1000003, 1280, 9)]
+
+
+def test_batch_matches_plain_path_and_replays(gpu_ctx):
+    want = [plain_reports(gpu_ctx, *r) for r in RANGES]
+    for _ in range(2):  # the second call replays the captured graph
+        sols, tot, st = gpu_ctx.search_batch(RANGES)
+        assert st.k_searched == sum(1 for a, b, _, _ in RANGES for K in range(a, b + 1) if will_search(K))
+        for i in range(len(RANGES)):
+            got = key([x for x in sols if x["range_id"] == i])
+            assert got == want[i], i
+            assert tot[i] == sum(f % 1000 + k for k, _, f in got)
+
+
+def test_batch_many_rounds_against_golden(gpu_ctx, golden_index):
+    """More K than one round holds (16): the graph is replayed back to back;
+    per-unit hash lines equal the reference's."""
+    units = [g for g in golden_index if g["kmin"] == g["kmax"]]
+    assert len(units) >= 12
+    units = units + units[:6]  # > 16 K: at least two rounds
+    ranges = [(g["kmin"], g["kmax"], g["shift"], 0) for g in units]
+    sols, tot, st = gpu_ctx.search_batch(ranges)
+    for i, g in enumerate(units):
+        recs, hline = read_sol(os.path.join(GOLDEN, "ref_min10", g["name"] + ".sol"))
+        assert H.hash_line(g["kmin"], g["kmax"], H.cap_total(int(tot[i]))) == hline, g["name"]
+        got = [r for r in key([x for x in sols if x["range_id"] == i]) if r[0] >= 10]
+        assert got == recs
+
+
+def test_batch_edge_cases(gpu_ctx):
+    sols, tot, st = gpu_ctx.search_batch([(29, 31, 0, 0)])  # 29, 31 skipped, 30 searched
+    assert st.k_searched == 1 and st.k_skipped == 2
+    sols, tot, st = gpu_ctx.search_batch([(366384, 366384, 0, 10)])  # resumed past the last shift
+    assert len(sols) == 0 and tot[0] == 0
+    with pytest.raises(gpu.Ap26Error):
+        gpu_ctx.search_batch([(5, 4, 0, 0), (0, 3, 0, 0)])
+
+
+def test_server_alone_and_in_mix(gpu_ctx):
+    units = [(875859, 875860, 640, 0), (366384, 366384, 0, 0), (875861, 875863, 640, 0),
+             (44121555, 44121555, 0, 0), (1000003, 1000003, 1280, 3)]
+    alone = []
+    with gpu.Server(gpu_ctx, max_units=8, mem_budget_bytes=3 << 30) as srv:
+        for u in units:
+            t = srv.submit(*u)
+            r = None
+            while r is None:
+                srv.pump()
+                r = srv.result(t)
+            alone.append((key(r[0]), r[1]))
+    with gpu.Server(gpu_ctx, max_units=8, mem_budget_bytes=3 << 30) as srv:
+        assert srv.stats()["k_per_round"] == 4  # 3 GiB / ~0.76 GB per in-flight K
+        tickets = [srv.submit(*u) for u in units[:3]]
+        srv.pump()                      # the first round runs with three units
+        tickets += [srv.submit(*u) for u in units[3:]]  # late arrivals join
+        res = {}
+        while len(res) < len(units):
+            srv.pump()
+            for i, t in enumerate(tickets):
+                if i not in res:
+                    r = srv.result(t)
+                    if r is not None:
+                        res[i] = r
+        for i in range(len(units)):
+            assert (key(res[i][0]), res[i][1]) == alone[i], i
+            assert res[i][2] > 0
+        st = srv.stats()
+        assert st["units_done"] == len(units) and st["active"] == 0
+    # the plain path agrees
+    for i, u in enumerate(units):
+        assert alone[i][0] == plain_reports(gpu_ctx, *u)
+
+
+def test_server_admission_limit(gpu_ctx):
+    with gpu.Server(gpu_ctx, max_units=2, mem_budget_bytes=1 << 30) as srv:
+        a = srv.submit(366384, 366384, 0)
+        b = srv.submit(366385, 366385, 0)
+        assert a is not None and b is not None
+        assert srv.submit(366386, 366386, 0) is None  # full until a unit finishes
+        done = set()
+        while len(done) < 2:
+            srv.pump()
+            for t in (a, b):
+                if t not in done and srv.result(t) is not None:
+                    done.add(t)
+        assert srv.submit(366386, 366386, 0) is not None
+        e = srv.submit(29, 29, 0)  # nothing to search: finished at once
+        assert srv.result(e) is not None

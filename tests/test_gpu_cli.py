@@ -98,3 +98,10 @@ def test_resume_mid_k_and_shift_checkpoints(golden_index, tmp_path):
 def test_bad_arguments(tmp_path):
     r = run([1], tmp_path)
     assert r.returncode == 1 and "Usage" in r.stdout
+
+
+def test_batch_flag_same_output(golden_index, tmp_path):
+    r = run([875858, 875860, 640, "-report", 10, "-batch"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    want = _units_hash(golden_index, ["c1_875859", "c1_875860"], 875858, 875860)
+    assert read_sol(str(tmp_path / "SOL-AP26.txt")) == want
