@@ -66,13 +66,18 @@ struct BatchState {
   unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) */
   DevSol *d_sols = nullptr;
   unsigned long long *d_hash = nullptr;
-  hipGraphExec_t graph = nullptr;
-  int graph_njobs = -1;      /* graph is valid for this per_round/chunk setup */
+  /* one round = graph A (n43 starts, tables, persistent sieve) then graph B
+     (walk, advance); events recorded between them time the sieve side */
+  hipGraphExec_t graph = nullptr;   /* A */
+  hipGraphExec_t graph_b = nullptr; /* B */
+  std::vector<hipEvent_t> ev;       /* 2 per timed round */
 };
 
 static void batch_state_free(BatchState *b) {
   if (!b) return;
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
+  for (auto &e : b->ev) (void)hipEventDestroy(e);
   (void)hipFree(b->d_jobs);
   (void)hipFree(b->d_rs);
   (void)hipFree(b->d_triples);
@@ -152,10 +157,9 @@ static int ensure_capacity(BatchState *b, int njobs, int nranges) {
     int cap = njobs < 1024 ? 1024 : njobs;
     if (hipMalloc(&b->d_jobs, sizeof(BatchJobDev) * cap) != hipSuccess) return AP26_E_NOMEM;
     b->max_jobs = cap;
-    if (b->graph) {
-      (void)hipGraphExecDestroy(b->graph);
-      b->graph = nullptr;
-    }
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
+    b->graph = b->graph_b = nullptr;
   }
   if (nranges > b->max_ranges) {
     (void)hipFree(b->d_hash);
@@ -163,17 +167,18 @@ static int ensure_capacity(BatchState *b, int njobs, int nranges) {
     int cap = nranges < 1024 ? 1024 : nranges;
     if (hipMalloc(&b->d_hash, sizeof(unsigned long long) * cap) != hipSuccess) return AP26_E_NOMEM;
     b->max_ranges = cap;
-    if (b->graph) {
-      (void)hipGraphExecDestroy(b->graph);
-      b->graph = nullptr;
-    }
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
+    b->graph = b->graph_b = nullptr;
   }
   return AP26_OK;
 }
 
-/* enqueue one round on ctx->stream (captured into the graph) */
-static int enqueue_round(ap26_ctx *ctx, BatchState *b) {
+/* enqueue one round on ctx->stream (captured into the graphs): part 0 =
+   n43 starts, tables, sieve; part 1 = walk, advance */
+static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part) {
   const int R = b->per_round;
+  if (part == 1) goto walk;
   hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, b->d_jobs,
                      b->d_rs, b->d_triples, b->d_n43);
   hipLaunchKernelGGL(batch_tables_kernel, dim3((541 + 63) / 64, NPT, R * AP26_NSHIFT), dim3(64), 0,
@@ -191,6 +196,8 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b) {
   p.chunk = b->chunk;
   p.nchunk = b->nchunk;
   hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
+  return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
+walk:
   BatchWalkArgs w;
   w.jobs = b->d_jobs;
   w.rs = b->d_rs;
@@ -209,16 +216,31 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b) {
   return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
 }
 
-static int ensure_graph(ap26_ctx *ctx, BatchState *b) {
-  if (b->graph) return AP26_OK;
+static int capture(ap26_ctx *ctx, BatchState *b, int part, hipGraphExec_t *out) {
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_round(ctx, b);
+  int rc = enqueue_round(ctx, b, part);
   hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
   if (rc || ce != hipSuccess) return AP26_E_DEVICE;
-  hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   return ie == hipSuccess ? AP26_OK : AP26_E_DEVICE;
+}
+
+static int ensure_graph(ap26_ctx *ctx, BatchState *b) {
+  if (b->graph && b->graph_b) return AP26_OK;
+  int rc = capture(ctx, b, 0, &b->graph);
+  if (!rc) rc = capture(ctx, b, 1, &b->graph_b);
+  return rc;
+}
+
+/* launch one round; with ev0/ev1 recorded around the sieve graph */
+static int launch_round(ap26_ctx *ctx, BatchState *b, hipEvent_t ev0, hipEvent_t ev1) {
+  if (ev0) HIPCHK(hipEventRecord(ev0, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  if (ev1) HIPCHK(hipEventRecord(ev1, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->graph_b, ctx->stream));
+  return AP26_OK;
 }
 
 static BatchJobDev make_job(int32_t K, int32_t shift, int32_t it0, int32_t range_id) {
@@ -301,8 +323,17 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   rc = ensure_graph(ctx, b);
   if (rc) return rc;
   const int rounds = (int)((jobs.size() + b->per_round - 1) / b->per_round);
+  const int timed = rounds < 1024 ? rounds : 1024;
+  while ((int)b->ev.size() < 2 * timed) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    b->ev.push_back(e);
+  }
   HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-  for (int i = 0; i < rounds; i++) HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  for (int i = 0; i < rounds; i++) {
+    rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
+    if (rc) return rc;
+  }
   HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
   unsigned long long cnt[8];
   HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
@@ -327,12 +358,17 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     for (int i = 0; i < nr; i++) hash_totals[i] = (int64_t)tot[i];
   st.prp_tests = (int64_t)cnt[1];
   st.reports = nsol;
-  st.ms_sieve = 0;
+  for (const BatchJobDev &j : jobs)
+    st.positions += (int64_t)AP26_NN43 * AP26_N53_PER_N43 * AP26_N59_STEPS * (AP26_NSHIFT - j.it0);
   {
+    /* ms_walk: device time of all rounds; ms_sieve: sum over the timed rounds
+       of the sieve graph (n43 starts + tables + persistent sieve) */
     float ms = 0;
-    if (hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) st.ms_sieve = ms;
+    if (hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) st.ms_walk = ms;
+    for (int i = 0; i < timed; i++)
+      if (hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]) == hipSuccess) st.ms_sieve += ms;
   }
-  st.sieve_launches = rounds;
+  st.sieve_launches = timed;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
@@ -500,7 +536,8 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   s->h_rs->pad = 0;
   HIPCHK(hipMemcpyAsync(b->d_jobs, s->h_jobs, sizeof(BatchJobDev) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  int rc = launch_round(ctx, b, nullptr, nullptr);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
                         ctx->stream));

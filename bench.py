@@ -1,23 +1,32 @@
 """Benchmark of the AP26 hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain|prp]
-                    [--no-cpu] [--variant N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--plain-steps N]
+                    [--no-cpu] [--no-serve] [--no-plain]
 
-Prints ONE JSON line.  A step is one searched K of the plain single-GPU
-workload: the full 10-shift search of that K (tables -> sieve -> SPRP walk ->
-AP check) plus the host-side ReportSolution/hash of its reports.  The K are the
-consecutive K from 48620635 (numpy default_rng(26).integers(4e7, 5e7+1)) at
-SHIFT 640 that pass will_search; skipped K are not steps.  Each step copies
-the K's 10840 n43 starts (86 KB) host->device, as the reference's plain GPU
-path does; everything else is generated on the device.
+Prints ONE JSON line.  The headline `value` is K/sec (10-shift) of the batched
+persistent path; a step is one searched K.  The timed K are a bounded sample
+of the batched workload: 256 units from numpy default_rng(26) with length
+floor(exp(U(0, ln 4096))), start uniform in [1e6, 1e8], SHIFT from
+{0, 640, 1280}; the sample takes the first min(length, 4) K of units in order
+until `steps` searched K are collected, so it mixes SHIFTs, unit lengths and
+K magnitudes.  The whole unit is searched -- tables, all 10 shifts of the
+sieve, the SPRP walk, AP check, per-range hash totals on the device -- and the
+host applies ReportSolution (9-base check, hash) to every report.
 
-Extra objects on the line:
-  roofline      the sieve kernel (the dominant one) priced on the INT32 VALU
-                roofline from algorithmic counts, timed live with HIP events;
+Beside it on the line:
+  plain         the plain per-K path (one kernel sequence per K, host sync per
+                K) on consecutive K from 48620635 at SHIFT 640;
+  roofline      the batched path's persistent sieve kernel (the dominant one)
+                priced on the INT32 VALU roofline from algorithmic counts,
+                timed with HIP events recorded inside the replayed graph;
   prp           the isolated SPRP micro-benchmark (2^26 odd values in
                 [2^58, 2^63)) and its INT32 roofline;
+  serving       continuous batching with Poisson arrivals at half the
+                saturated unit rate, unit length floor(exp(U(0, ln 16))) (the
+                workload's 1..8192 is scaled down so that units finish within
+                the bench), p50/p95/p99 unit latency;
   cpu_baseline  the oracle (a from-scratch C restatement of the reference CPU
-                search) on a bounded sample of the same workload, all host
+                search) on a bounded sample of the plain workload, on the host
                 cores the box grants (up to 16).
 """
 import argparse
@@ -38,12 +47,15 @@ from ap26_amd import host as H  # noqa: E402
 METRIC = "K/sec (10-shift) + PRP tests/sec, batched vs plain GPU; p95 work-unit latency"
 K0_C3 = 48620635          # default_rng(26).integers(40_000_000, 50_000_001)
 SHIFT_C3 = 640
-N59_PER_K = 10840 * 12673 * 35  # positions per shift per K
+MASKS_PER_SHIFT = 10840 * 12673 * 35  # n59 positions per shift per K
 # Lookups per 64-bit mask evaluation under the reference GPU kernel's
 # early-exit grouping (src/opencl/kernels/sieve.cl:32-114), measured with the
-# oracle's ap26o_lookup_count over 40 n43 starts spread across K=48620635,
-# SHIFT 640 (all 10 shifts): 19.466.
+# oracle's ap26o_lookup_count over 40 n43 starts spread across K=48620635 at
+# SHIFT 640 (all 10 shifts): 19.466; over K=366384 at SHIFT 0: 18.224; over
+# K=875859 at SHIFT 640: 18.423.  The batched sample mixes K from 1e6 to 1e8
+# and uses the mean of the three.
 LOOKUPS_PER_MASK_C3 = 19.466
+LOOKUPS_PER_MASK_MIX = (19.466 + 18.224 + 18.423) / 3
 # INT32 op-equivalents per lookup: quotient estimate + multiply-subtract
 # (residue), AND of the 64-bit mask (2).  The table load itself is not counted.
 OPS_PER_LOOKUP = 4
@@ -56,14 +68,65 @@ def will_search(K):
     return all(K % p for p in (29, 31, 37, 41, 43, 47, 53, 59))
 
 
-def c3_ks(n, start=0):
-    out = []
-    K = K0_C3
-    while len(out) < start + n:
+def c3_ks(n):
+    out, K = [], K0_C3
+    while len(out) < n:
         if will_search(K):
             out.append(K)
         K += 1
-    return out[start:]
+    return out
+
+
+def c4_units(seed=26, n=256):
+    rng = np.random.default_rng(seed)
+    lens = np.clip(np.floor(np.exp(rng.uniform(0, np.log(4096), n))).astype(int), 1, 4096)
+    starts = rng.integers(1_000_000, 100_000_001, n)
+    shifts = rng.choice([0, 640, 1280], n)
+    return [(int(s), int(s + L - 1), int(sh)) for s, L, sh in zip(starts, lens, shifts)]
+
+
+def c4_sample(steps, per_unit=4):
+    """Units (kmin, kmax, shift, 0) covering the first `steps` searched K."""
+    out, got = [], 0
+    for kmin, kmax, sh in c4_units():
+        ks = [K for K in range(kmin, kmax + 1) if will_search(K)][:per_unit]
+        if not ks:
+            continue
+        ks = ks[:steps - got]
+        out.append((ks[0], ks[-1], sh, 0))
+        got += len([K for K in range(ks[0], ks[-1] + 1) if will_search(K)])
+        if got >= steps:
+            break
+    return out, got
+
+
+def report_all(hosts, sols):
+    for s in sorted(sols.tolist(), key=lambda s: (s[3], s[2], s[0])):
+        h = hosts[s[3]] if s[3] >= 0 else hosts[0]
+        if h.report(s[0], s[1], s[2]) != 0:
+            raise RuntimeError("report failed")
+
+
+def run_batched(ctx, steps, warmup):
+    wu, _ = c4_sample(warmup, per_unit=1)
+    ctx.search_batch(wu)
+    units, nk = c4_sample(steps)
+    tmp = tempfile.mkdtemp(prefix="ap26bench")
+    hosts = []
+    for i, (a, b, sh, _) in enumerate(units):
+        d = os.path.join(tmp, str(i))
+        os.makedirs(d)
+        hosts.append(H.Host(a, b, sh, d, min_report=20))
+    ctx.sync()
+    t0 = time.perf_counter()
+    sols, tot, st = ctx.search_batch(units)
+    report_all(hosts, sols)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    for h in hosts:
+        h.close()
+    return {"seconds": dt, "k": nk, "units": len(units), "stats": st.as_dict(),
+            "shifts": sorted(set(u[2] for u in units))}
 
 
 def run_plain(ctx, steps, warmup):
@@ -71,14 +134,9 @@ def run_plain(ctx, steps, warmup):
     tmp = tempfile.mkdtemp(prefix="ap26bench")
     h = H.Host(ks[0], ks[-1], SHIFT_C3, tmp, min_report=20)
     for K in ks[:warmup]:
-        sols, _ = ctx.search_k(K, SHIFT_C3)
+        ctx.search_k(K, SHIFT_C3)
     ctx.sync()
-    sieve_ms = 0.0
-    launches = 0
-    positions = 0
-    surv = 0
-    tests = 0
-    walk_ms = 0.0
+    agg = {"sieve_ms": 0.0, "walk_ms": 0.0, "launches": 0, "positions": 0, "survivors": 0, "tests": 0}
     per_k = []
     t0 = time.perf_counter()
     for K in ks[warmup:]:
@@ -88,63 +146,105 @@ def run_plain(ctx, steps, warmup):
             if h.report(s[0], s[1], s[2]) != 0:
                 raise RuntimeError("report failed")
         per_k.append(time.perf_counter() - tk)
-        sieve_ms += st.ms_sieve
-        walk_ms += st.ms_walk
-        launches += st.sieve_launches
-        positions += st.positions
-        surv += st.survivors
-        tests += st.prp_tests
+        agg["sieve_ms"] += st.ms_sieve
+        agg["walk_ms"] += st.ms_walk
+        agg["launches"] += st.sieve_launches
+        agg["positions"] += st.positions
+        agg["survivors"] += st.survivors
+        agg["tests"] += st.prp_tests
     ctx.sync()
     dt = time.perf_counter() - t0
     h.close()
-    return {"seconds": dt, "ks": ks[warmup:], "sieve_ms": sieve_ms, "walk_ms": walk_ms,
-            "launches": launches, "positions": positions, "survivors": surv, "prp_tests": tests,
-            "per_k_s": per_k, "hash": h.hash}
+    n = len(ks) - warmup
+    avg_launch_s = agg["sieve_ms"] / agg["launches"] / 1e3
+    ops = agg["positions"] / agg["launches"] * LOOKUPS_PER_MASK_C3 * OPS_PER_LOOKUP
+    return {"k_per_s": n / dt, "k": n, "ms_per_k": dt * 1e3 / n, "K_first": ks[warmup], "K_last": ks[-1],
+            "sieve_ms_per_k": agg["sieve_ms"] / n, "walk_ms_per_k": agg["walk_ms"] / n,
+            "survivors_per_k": agg["survivors"] / n, "prp_tests_per_k": agg["tests"] / n,
+            "p95_k_latency_ms": float(np.percentile(per_k, 95) * 1e3),
+            "sieve_roofline": {"achieved": ops / avg_launch_s / 1e12, "avg_launch_ms": avg_launch_s * 1e3,
+                               "frac": ops / avg_launch_s / 1e12 / PEAK_INT32_TOPS}}
 
 
 def run_prp(ctx, count=1 << 26, seed=26):
     rng = np.random.default_rng(seed)
     n = rng.integers(2 ** 58, 2 ** 63, size=count, dtype=np.uint64) | np.uint64(1)
-    # W(N) = bitlen(N) - 2 squarings, counted exactly on the inputs
     bl = np.floor(np.log2(n.astype(np.float64))).astype(np.int64) + 1
-    squarings = int((bl - 2).sum())
+    squarings = int((bl - 2).sum())  # W(N) = bitlen(N) - 2, counted on the inputs
     d_n = ctx.dev_alloc(n.nbytes)
     d_o = ctx.dev_alloc(count)
     ctx.h2d(d_n, n)
-    ctx.prp_batch_dev(d_n, d_o, count)  # warm-up
+    ctx.prp_batch_dev(d_n, d_o, count)
     times = [ctx.prp_batch_dev(d_n, d_o, count) for _ in range(5)]
-    out = np.zeros(count, dtype=np.uint8)
-    ctx.d2h(out, d_o)
     ctx.dev_free(d_n)
     ctx.dev_free(d_o)
     ms = float(np.median(times))
-    ops = squarings * OPS_PER_SQUARING
-    return {"tests": count, "ms": ms, "tests_per_s": count / (ms / 1e3),
-            "int32_tops": ops / (ms / 1e3) / 1e12, "prp_count": int(out.sum()),
-            "mean_squarings": squarings / count}
+    tops = squarings * OPS_PER_SQUARING / (ms / 1e3) / 1e12
+    return {"tests_per_s": count / (ms / 1e3), "ms": ms, "tests": count,
+            "mean_squarings": squarings / count,
+            "roofline": {"bound": "valu_int32", "achieved": tops, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
+                         "frac": tops / PEAK_INT32_TOPS}}
+
+
+def run_serving(ctx, kps, n_units=12, budget=3 << 30, seed=27):
+    rng = np.random.default_rng(seed)
+    lens = np.floor(np.exp(rng.uniform(0, np.log(16), n_units))).astype(int)
+    starts = rng.integers(1_000_000, 100_000_001, n_units)
+    shifts = rng.choice([0, 640, 1280], n_units)
+    units = [(int(s), int(s + L - 1), int(sh)) for s, L, sh in zip(starts, lens, shifts)]
+    mean_k = np.mean([sum(will_search(K) for K in range(a, b + 1)) for a, b, _ in units])
+    rate = 0.5 * kps / max(mean_k, 1e-9)   # units per second
+    gaps = rng.exponential(1.0 / rate, n_units)
+    arrive = np.cumsum(gaps) - gaps[0]
+    lat = []
+    with gpu.Server(ctx, max_units=64, mem_budget_bytes=budget) as srv:
+        t0 = time.perf_counter()
+        pending = list(range(n_units))
+        tickets = {}
+        while len(lat) < n_units:
+            now = time.perf_counter() - t0
+            while pending and arrive[pending[0]] <= now:
+                i = pending.pop(0)
+                tickets[i] = srv.submit(*units[i])
+            if srv.stats()["active"] == 0 and pending:
+                time.sleep(max(0.0, arrive[pending[0]] - (time.perf_counter() - t0)))
+                continue
+            srv.pump()
+            for i, t in list(tickets.items()):
+                r = srv.result(t)
+                if r is not None:
+                    lat.append(r[2])
+                    del tickets[i]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    lat = np.array(lat)
+    return {"units": n_units, "rate_units_per_s": rate, "load": 0.5, "k_per_s": st["k_done"] / wall,
+            "k_per_round": st["k_per_round"], "p50_ms": float(np.percentile(lat, 50)),
+            "p95_ms": float(np.percentile(lat, 95)), "p99_ms": float(np.percentile(lat, 99)),
+            "unit_len": "floor(exp(U(0, ln 16)))"}
 
 
 def cpu_baseline(threads, j1=2048):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    K = K0_C3
     t = time.perf_counter()
-    O.search_span(K, SHIFT_C3, 0, 10, 0, j1, threads=threads)
+    O.search_span(K0_C3, SHIFT_C3, 0, 10, 0, j1, threads=threads)
     dt = time.perf_counter() - t
     frac = j1 / 10840
     return {"value": frac / dt, "unit": "K/s", "cores": threads, "kind": "port",
-            "sample": f"oracle search of K={K} SHIFT {SHIFT_C3}, all 10 shifts, n43 starts "
+            "sample": f"oracle search of K={K0_C3} SHIFT {SHIFT_C3}, all 10 shifts, n43 starts "
                       f"0..{j1 - 1} ({frac * 100:.1f}% of one K) in {dt:.1f} s"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=48)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="plain", choices=["plain"])
-    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--plain-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -152,51 +252,48 @@ def main():
               "(the search shards by K with no inter-GPU traffic; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    ctx = gpu.Context(0, sieve_variant=args.variant)
-    r = run_plain(ctx, args.steps, args.warmup)
-    kps = len(r["ks"]) / r["seconds"]
-    skipped = (r["ks"][-1] - r["ks"][0] + 1) - len(r["ks"])
-    avg_launch_s = r["sieve_ms"] / r["launches"] / 1e3
-    pos_per_launch = r["positions"] / r["launches"]
-    ops_per_launch = pos_per_launch * LOOKUPS_PER_MASK_C3 * OPS_PER_LOOKUP
-    achieved = ops_per_launch / avg_launch_s / 1e12
+    ctx = gpu.Context(0)
+    b = run_batched(ctx, args.steps, args.warmup)
+    kps = b["k"] / b["seconds"]
+    st = b["stats"]
+    rounds = max(1, st["sieve_launches"])
+    pos_per_launch = st["positions"] / rounds
+    avg_launch_ms = st["ms_sieve"] / rounds
+    ops = pos_per_launch * LOOKUPS_PER_MASK_MIX * OPS_PER_LOOKUP
+    achieved = ops / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else None
     line = {
         "metric": METRIC,
         "value": kps,
         "unit": "K/s",
         "n_gpus": 1,
-        "steps": len(r["ks"]),
+        "steps": b["k"],
         "warmup": args.warmup,
-        "ms_per_step": r["seconds"] * 1e3 / len(r["ks"]),
+        "ms_per_step": b["seconds"] * 1e3 / b["k"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int64",
         "data": "synthetic",
-        "config": {"workload": "plain single-GPU path: full 10-shift search of consecutive K from "
-                               f"{K0_C3}, SHIFT {SHIFT_C3}, one kernel sequence per K",
-                   "K_first": r["ks"][0], "K_last": r["ks"][-1], "k_searched": len(r["ks"]),
-                   "k_skipped": skipped, "shift": SHIFT_C3, "sieve_variant": args.variant,
-                   "p95_k_latency_ms": float(np.percentile(r["per_k_s"], 95) * 1e3)},
-        "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": PEAK_INT32_TOPS,
-                     "unit": "TOP/s", "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
-                     "kernel": "sieve_queue_kernel" if args.variant == 0 else "sieve_kernel",
-                     "avg_launch_ms": avg_launch_s * 1e3,
-                     "ops_per_launch": ops_per_launch,
-                     "basis": f"{LOOKUPS_PER_MASK_C3} lookups/mask x {OPS_PER_LOOKUP} int32 ops "
-                              f"x {pos_per_launch:.4g} masks/launch"},
-        "detail": {"sieve_ms_per_k": r["sieve_ms"] / len(r["ks"]),
-                   "walk_ms_per_k": r["walk_ms"] / len(r["ks"]),
-                   "survivors_per_k": r["survivors"] / len(r["ks"]),
-                   "prp_tests_per_k": r["prp_tests"] / len(r["ks"])},
+        "config": {"workload": "batched persistent path: first min(len,4) K of the 256-unit mixed "
+                               "workload (default_rng(26); len floor(exp(U(0,ln 4096))), start U[1e6,1e8], "
+                               "SHIFT {0,640,1280}), one captured hipGraph per round of 16 K",
+                   "k_searched": b["k"], "units": b["units"], "shifts": b["shifts"],
+                   "rounds": rounds, "path": "batched"},
+        "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
+                     "frac": achieved / PEAK_INT32_TOPS if achieved else None, "traffic": None,
+                     "kernel": "sieve_persistent_kernel", "avg_launch_ms": avg_launch_ms,
+                     "ops_per_launch": ops,
+                     "basis": f"{LOOKUPS_PER_MASK_MIX:.3f} lookups/mask x {OPS_PER_LOOKUP} int32 ops x "
+                              f"{pos_per_launch:.4g} masks/launch (one launch = one round)"},
     }
+    if not args.no_plain:
+        p = run_plain(ctx, args.plain_steps, args.warmup)
+        line["plain"] = p
+        line["batched_over_plain"] = kps / p["k_per_s"]
     if not args.no_prp:
-        p = run_prp(ctx)
-        line["prp"] = {"tests_per_s": p["tests_per_s"], "ms": p["ms"], "tests": p["tests"],
-                       "mean_squarings": p["mean_squarings"],
-                       "roofline": {"bound": "valu_int32", "achieved": p["int32_tops"],
-                                    "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
-                                    "frac": p["int32_tops"] / PEAK_INT32_TOPS}}
+        line["prp"] = run_prp(ctx)
+    if not args.no_serve:
+        line["serving"] = run_serving(ctx, kps)
     ctx.close()
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))

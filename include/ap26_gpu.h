@@ -64,8 +64,10 @@ typedef struct {
   int64_t prp_tests;        /* base-2 SPRP tests run by the walk */
   int64_t reports;          /* (k >= 10) reports */
   double ms_tables;         /* table-build kernel time */
-  double ms_sieve;          /* sieve kernel time */
-  double ms_walk;           /* walk kernel time */
+  double ms_sieve;          /* sieve kernel time (batch: the last round's
+                               persistent sieve launch) */
+  double ms_walk;           /* walk kernel time (batch: device time of all
+                               rounds) */
   double ms_total;          /* wall time of the call */
   int32_t sieve_launches;
   int32_t walk_launches;
