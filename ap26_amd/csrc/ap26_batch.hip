@@ -28,21 +28,33 @@ constexpr int MAX_RANGES = 1 << 20;
 using PersistFn = void (*)(PersistArgs);
 
 struct PersistVariant {
-  int NL, waves;
+  int NL, waves, S;
   PersistFn fn;
 };
 
 template <int NL, int WAVES, class LV>
 PersistVariant make_pvariant() {
-  return PersistVariant{NL, WAVES, &sieve_persistent_kernel<NL, WAVES, LV>};
+  return PersistVariant{NL, WAVES, 1, &sieve_persistent_kernel<NL, WAVES, LV>};
 }
 
+template <int NL, int WAVES, class LV>
+PersistVariant make_pvariant2() {
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV>};
+}
+
+/* 1..3, 9: one shift per item; 0 (default), 10..14: shift pairs */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
     case 2: return make_pvariant<45, 12, Levels4>();
     case 3: return make_pvariant<25, 16, Levels5>();
-    default: return make_pvariant<38, 16, Levels4>();
+    case 10: return make_pvariant2<29, 16, Levels4>();
+    case 11: return make_pvariant2<31, 14, Levels4>();
+    case 12: return make_pvariant2<35, 12, Levels4>();
+    case 13: return make_pvariant2<20, 12, Levels5>();
+    case 14: return make_pvariant2<31, 12, Levels4>();
+    case 9: return make_pvariant<38, 16, Levels4>();
+    default: return make_pvariant2<29, 16, Levels4>();
   }
 }
 
@@ -103,12 +115,13 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
   if (!b) return AP26_E_NOMEM;
   int rc = AP26_OK;
   b->per_round = per_round;
+  /* environment overrides > autotuned choice > defaults */
   const char *e = getenv("AP26_BATCH_CHUNK");
-  b->chunk = e ? atoi(e) : 85;
+  b->chunk = e ? atoi(e) : (ctx->tune_chunk > 0 ? ctx->tune_chunk : 85);
   if (b->chunk < 1 || b->chunk > AP26_NN43) b->chunk = 85;
   b->nchunk = (AP26_NN43 + b->chunk - 1) / b->chunk;
   e = getenv("AP26_BATCH_VARIANT");
-  b->pvariant = e ? atoi(e) : 0;
+  b->pvariant = e ? atoi(e) : (ctx->tune_variant >= 0 ? ctx->tune_variant : 0);
   b->pv = pick_pvariant(b->pvariant);
   b->surv_cap = (int64_t)b->per_round * (INT64_C(60) << 20); /* ~45M survivors per K */
   /* guarded (i31, i37, i41) triples in the reference order */
@@ -137,7 +150,6 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
     batch_state_free(b);
     return rc;
   }
-  (void)ctx;
   *out = b;
   return AP26_OK;
 }
@@ -182,7 +194,7 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part) {
   hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, b->d_jobs,
                      b->d_rs, b->d_triples, b->d_n43);
   hipLaunchKernelGGL(batch_tables_kernel, dim3((541 + 63) / 64, NPT, R * AP26_NSHIFT), dim3(64), 0,
-                     ctx->stream, b->d_jobs, b->d_rs, b->d_tab);
+                     ctx->stream, b->d_jobs, b->d_rs, b->d_tab, b->pv.S);
   PersistArgs p;
   p.jobs = b->d_jobs;
   p.rs = b->d_rs;
@@ -614,4 +626,86 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
   if (k_per_round) *k_per_round = s->R;
   if (active) *active = (int32_t)s->active.size();
   return AP26_OK;
+}
+
+/* ---- autotuner ------------------------------------------------------------ */
+/* Times the batched engine's sieve variants (LDS primes, waves per CU,
+   compaction levels, one shift or a shift pair per item) and work-item sizes
+   (n43 starts per item) on one K, keeps the fastest for this context and, if
+   a path is given, writes it to a cache file that ap26_open loads again when
+   AP26_TUNE_CACHE names it. */
+
+static const int TUNE_VARIANTS[] = {10, 11, 12, 13, 14, 9, 1, 2};
+static const int TUNE_CHUNKS[] = {43, 85, 170};
+
+extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,
+                             ap26_tune_entry *results, int32_t cap, int32_t *n_out, ap26_tune_entry *best) {
+  if (!ctx || !host_will_search(K) || SHIFT < 0 || SHIFT > (1 << 20)) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  const int nv = quick ? 5 : (int)(sizeof(TUNE_VARIANTS) / sizeof(int));
+  const int nc = quick ? 2 : (int)(sizeof(TUNE_CHUNKS) / sizeof(int));
+  const int save_v = ctx->tune_variant, save_c = ctx->tune_chunk;
+  BatchState *save_b = ctx->batch;
+  ap26_tune_entry top{-1, -1, 1e30};
+  int n = 0, rc = AP26_OK;
+  ap26_range r{K, K, SHIFT, 0};
+  ap26_sol *sols = (ap26_sol *)malloc(sizeof(ap26_sol) * 4096);
+  if (!sols) return AP26_E_NOMEM;
+  for (int vi = 0; vi < nv && !rc; vi++)
+    for (int ci = 0; ci < nc && !rc; ci++) {
+      ctx->tune_variant = TUNE_VARIANTS[vi];
+      ctx->tune_chunk = quick ? TUNE_CHUNKS[ci + 1] : TUNE_CHUNKS[ci];
+      BatchState *tmp = nullptr;
+      rc = batch_state_new(ctx, 1, &tmp);
+      if (rc) break;
+      ctx->batch = tmp;
+      int32_t ns = 0;
+      int64_t tot = 0;
+      ap26_stats st{};
+      rc = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st); /* warm: builds the graphs */
+      if (!rc) rc = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st);
+      ctx->batch = save_b;
+      batch_state_free(tmp);
+      if (rc) break;
+      ap26_tune_entry t{ctx->tune_variant, ctx->tune_chunk, st.ms_walk};
+      if (results && n < cap) results[n] = t;
+      n++;
+      if (t.ms_per_k < top.ms_per_k) top = t;
+    }
+  free(sols);
+  ctx->batch = save_b;
+  if (rc) {
+    ctx->tune_variant = save_v;
+    ctx->tune_chunk = save_c;
+    return rc;
+  }
+  ctx->tune_variant = top.variant;
+  ctx->tune_chunk = top.chunk;
+  batch_free(ctx); /* the next batch is built with the tuned choice */
+  if (n_out) *n_out = n;
+  if (best) *best = top;
+  if (cache_path && cache_path[0]) {
+    FILE *f = fopen(cache_path, "w");
+    if (!f) return AP26_E_RANGE;
+    fprintf(f, "ap26-tune 1 %s %d %d %.3f\n", ctx->name, top.variant, top.chunk, top.ms_per_k);
+    fclose(f);
+  }
+  return AP26_OK;
+}
+
+/* called by ap26_open: adopt a cached choice for this device */
+void tune_load_cache(ap26_ctx *ctx) {
+  const char *path = getenv("AP26_TUNE_CACHE");
+  if (!path || !path[0]) return;
+  FILE *f = fopen(path, "r");
+  if (!f) return;
+  char arch[256];
+  int ver = 0, v = -1, c = -1;
+  double ms = 0;
+  if (fscanf(f, "ap26-tune %d %255s %d %d %lf", &ver, arch, &v, &c, &ms) == 5 && ver == 1 &&
+      strcmp(arch, ctx->name) == 0 && c > 0 && c <= AP26_NN43) {
+    ctx->tune_variant = v;
+    ctx->tune_chunk = c;
+  }
+  fclose(f);
 }

@@ -41,9 +41,9 @@ SieveVariant make_variant() {
   return SieveVariant{S, NL, 256, &sieve_kernel<S, NL>};
 }
 
-template <int NL, int WAVES, class LV, bool VC>
+template <int NL, int WAVES, class LV, bool VC, unsigned G = 0>
 SieveVariant make_qvariant() {
-  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC>};
+  return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC, G>};
 }
 
 
@@ -65,6 +65,8 @@ struct ap26_ctx {
   DevSol *d_sols = nullptr;      /* SOL_CAP */
   SieveVariant sieve{};
   BatchState *batch = nullptr;
+  int tune_variant = -1; /* batched sieve variant / work-item chunk chosen by */
+  int tune_chunk = -1;   /* ap26_autotune (or its cache); -1 = default       */
 };
 
 
@@ -73,5 +75,6 @@ SieveVariant pick_variant(int id);
 /* batched engine state (ap26_batch.hip) */
 struct BatchState;
 void batch_free(ap26_ctx *ctx);
+void tune_load_cache(ap26_ctx *ctx);
 
 #endif

@@ -21,7 +21,7 @@ using namespace ap26;
 /* Sieve variants: 0 = default queued sieve, 1 = simple S=1, 2 = simple S=2,
    10+ = queued-sieve tuning candidates (LDS primes, waves per CU, levels, VGPR
    constants). */
-#define AP26_NVARIANTS 26
+#define AP26_NVARIANTS 31
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -42,6 +42,11 @@ SieveVariant pick_variant(int id) {
     case 23: return make_qvariant<38, 16, Levels4, false>();
     case 24: return make_qvariant<25, 16, Levels5, false>();
     case 25: return make_qvariant<31, 12, Levels5, false>();
+    case 26: return make_qvariant<38, 16, Levels4, false, 0x5555>();
+    case 27: return make_qvariant<38, 16, Levels4, false, 0x1111>();
+    case 28: return make_qvariant<38, 16, Levels4, false, 0x7fff>();
+    case 29: return make_qvariant<38, 16, Levels4, false, 0x0fff>();
+    case 30: return make_qvariant<38, 16, Levels4, false, 0x00ff>();
     default: return make_qvariant<38, 16, Levels4, false>();
   }
 }
@@ -93,6 +98,7 @@ int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
       break;
     }
     ctx->sieve = pick_variant(ctx->opts.sieve_variant);
+    tune_load_cache(ctx);
     ctx->opts.block = ctx->sieve.block;
     if (ctx->opts.blocks_per_cu <= 0) ctx->opts.blocks_per_cu = ctx->sieve.block > 256 ? 1 : 2;
   } while (0);

@@ -263,7 +263,7 @@ struct VConsts {
   }
 };
 
-template <int NL, int B, int E>
+template <int NL, int B, int E, unsigned G = 0>
 __device__ __forceinline__ uint64_t apply_primes(uint64_t m, double xd, const char *ldsb, const char *gtb) {
   const double x = xd - TWO52;
   const uint32_t xs = lo32(xd) * 8u;
@@ -277,7 +277,7 @@ __device__ __forceinline__ uint64_t apply_primes(uint64_t m, double xd, const ch
     constexpr int I = decltype(IC)::value;
     constexpr int P = PT.p[I];
     constexpr int C0 = 8 * (PT.off[I] + (P - 1) / 2);
-    if constexpr (I < NL)
+    if constexpr (I < NL && !(I < 32 && ((G >> I) & 1)))
       w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
     else
       w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
@@ -423,7 +423,7 @@ constexpr int qsieve_lds_bytes() {
 
 /* The positions of n53 blocks [blk0, blk1) taken by this thread at tid,
    tid+stride, ...: level 0 on every lane, queues for the rest, then flush. */
-template <int NL, class LV, bool VC>
+template <int NL, class LV, bool VC, unsigned G = 0>
 __device__ __forceinline__ void qsieve_blocks(QSieve<NL, LV> &qs, const VConsts<0, LV::B[1]> &vc,
                                               const SieveArgs &a, int64_t tid, int64_t stride) {
   const double S59d = (double)a.kp.S59;
@@ -439,7 +439,7 @@ __device__ __forceinline__ void qsieve_blocks(QSieve<NL, LV> &qs, const VConsts<
       if constexpr (VC)
         m = apply_primes_vc<0, LV::B[1]>(~(uint64_t)0, xd, qs.ldsb, vc);
       else
-        m = apply_primes<NL, 0, LV::B[1]>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
+        m = apply_primes<NL, 0, LV::B[1], G>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
     }
     qs.push(0, xd, m, has && m != 0);
     if (has) {
@@ -463,8 +463,10 @@ struct QSieveLds {
   QEntry queues[WAVES][LV::N - 1][128];
 };
 
-/* One launch per (K, shift): the plain path.  VC: level-0 constants in VGPRs. */
-template <int NL, int WAVES, class LV, bool VC>
+/* One launch per (K, shift): the plain path.  VC: level-0 constants in VGPRs;
+   G: level-0 primes (bit i = prime index i) read through the vector L1 cache
+   instead of LDS, so that the two gather paths work in parallel. */
+template <int NL, int WAVES, class LV, bool VC, unsigned G = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, WAVES / 4)))
 sieve_queue_kernel(SieveArgs a) {
   static_assert(qsieve_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
@@ -484,8 +486,8 @@ sieve_queue_kernel(SieveArgs a) {
   qs.a = &a;
   VConsts<0, LV::B[1]> vc;
   if constexpr (VC) vc.init();
-  qsieve_blocks<NL, LV, VC>(qs, vc, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
-                            (int64_t)gridDim.x * blockDim.x);
+  qsieve_blocks<NL, LV, VC, G>(qs, vc, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                               (int64_t)gridDim.x * blockDim.x);
 }
 
 /* ---- batched persistent sieve ------------------------------------------ */
@@ -601,9 +603,11 @@ static __global__ void n43_kernel(const BatchJobDev *__restrict__ jobs, const Ro
   n43[(size_t)jl * AP26_NN43 + j] = v;
 }
 
-/* Tables of every (job, shift) of the round; grid z = job*10 + shift. */
+/* Tables of every (job, shift) of the round; grid z = job*10 + shift.  S = 1:
+   [job][shift][entry]; S = 2: [job][shift pair][entry][shift in pair].  Shifts
+   before the job's resume point get all-zero masks. */
 static __global__ void batch_tables_kernel(const BatchJobDev *__restrict__ jobs, const RoundState *__restrict__ rsp,
-                                    uint64_t *__restrict__ tab) {
+                                           uint64_t *__restrict__ tab, int S) {
   const RoundState rs = *rsp;
   const int njobs = min(rs.per_round, rs.njobs - rs.base);
   const int jl = blockIdx.z / AP26_NSHIFT;
@@ -612,8 +616,281 @@ static __global__ void batch_tables_kernel(const BatchJobDev *__restrict__ jobs,
   const int ti = blockIdx.y;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= PT.p[ti]) return;
-  const KParams &kp = jobs[rs.base + jl].kp;
-  tab[((size_t)jl * AP26_NSHIFT + it) * NTAB + PT.off[ti] + c] = table_word(kp, ti, c, kp.SHIFT + 64 * it);
+  const BatchJobDev &job = jobs[rs.base + jl];
+  const uint64_t w = it < job.it0 ? 0 : table_word(job.kp, ti, c, job.kp.SHIFT + 64 * it);
+  if (S == 1)
+    tab[((size_t)jl * AP26_NSHIFT + it) * NTAB + PT.off[ti] + c] = w;
+  else
+    tab[((size_t)jl * (AP26_NSHIFT / 2) + it / 2) * NTAB * 2 + (size_t)(PT.off[ti] + c) * 2 + (it & 1)] = w;
+}
+
+/* ---- shift-pair persistent sieve (S = 2) -------------------------------- */
+/* Level 0 reads both shifts of a pair with one 16-byte LDS load per prime and
+   one residue; the surviving (position, shift) pairs are queued separately,
+   the shift bit stored in bit 48 of the position's double (n59 < 2^48). */
+
+constexpr double TWO48 = 281474976710656.0;
+
+__device__ __forceinline__ void decode2(double xd, double &x, uint32_t &xs, int &e) {
+  e = (int)((__double_as_longlong(xd) >> 48) & 1);
+  x = xd - (e ? TWO52 + TWO48 : TWO52);
+  xs = lo32(xd) * 16u + 8u * (uint32_t)e;
+}
+
+template <int NL, int B, int E>
+__device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const char *ldsb, const char *gtb) {
+  double x;
+  uint32_t xs;
+  int e;
+  decode2(xd, x, xs, e);
+  int bo[E - B];
+  uint64_t w[E - B];
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    bo[I - B] = centred_byteoff<PT.p[I], 16>(x, xs);
+  });
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
+    if constexpr (I < NL)
+      w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
+    else
+      w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
+  });
+  static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
+  return m;
+}
+
+template <int NL>
+__device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const char *ldsb) {
+  double x;
+  uint32_t xs;
+  int e;
+  decode2(xd, x, xs, e);
+  constexpr int LMAIN = PT.off[NL];
+  static_for<NSP, NPT>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    const int bo = centred_byteoff<PT.p[I], 16>(x, xs);
+    constexpr int C0 = 16 * (LMAIN + PT.off[I] - NOK + (PT.p[I] - 1) / 2);
+    m &= *(const uint64_t *)(ldsb + C0 + bo);
+  });
+  return m;
+}
+
+/* level 0 of a fresh position (shift bit 0): both masks of the pair */
+template <int B, int E>
+__device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_t &m0, uint64_t &m1) {
+  const double x = xd - TWO52;
+  const uint32_t xs = lo32(xd) * 16u;
+  int bo[E - B];
+  uint4 w[E - B];
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    bo[I - B] = centred_byteoff<PT.p[I], 16>(x, xs);
+  });
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
+    w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
+  });
+  uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
+  static_for<B, E>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    a0 &= w[I - B].x;
+    a1 &= w[I - B].y;
+    a2 &= w[I - B].z;
+    a3 &= w[I - B].w;
+  });
+  m0 = ((uint64_t)a1 << 32) | a0;
+  m1 = ((uint64_t)a3 << 32) | a2;
+}
+
+__device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const SieveArgs &a, int lane) {
+  const uint64_t bits = (uint64_t)__double_as_longlong(xd);
+  const int e = (int)((bits >> 48) & 1);
+  const int64_t n59 = (int64_t)(bits & ((INT64_C(1) << 48) - 1));
+  const int64_t nbase = n59 + (int64_t)(a.shift0 + 64 * e) * AP26_MOD;
+  while (true) {
+    const bool has = w != 0;
+    const uint64_t bal = __ballot(has);
+    if (bal == 0) break;
+    const int cnt = __popcll(bal);
+    const int leader = __ffsll((unsigned long long)bal) - 1;
+    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(a.nsurv, (unsigned long long)cnt);
+    base = __shfl(base, leader);
+    if (has) {
+      const int bit = __ffsll((unsigned long long)w) - 1;
+      w &= w - 1;
+      const unsigned long long idx = base + (unsigned long long)pre;
+      if (idx < (unsigned long long)a.cap) {
+        a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
+        if (a.surv_job) a.surv_job[idx] = a.job;
+      }
+    }
+  }
+}
+
+template <int NL, class LV>
+struct QSieve2 {
+  static constexpr int QCAP = 128;
+  static constexpr int NQ = LV::N - 1;
+  QEntry (*q)[QCAP];
+  int qn[NQ];
+  const char *ldsb, *gtb;
+  int lane;
+  const SieveArgs *a;
+
+  __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
+    const uint64_t bal = __ballot(pred);
+    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
+    qn[qi] += __popcll(bal);
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  template <int L>
+  __device__ __forceinline__ void run_level(const QEntry &e, bool pred) {
+    uint64_t m = e.m;
+    if (pred) m = apply_primes2<NL, LV::B[L + 1], LV::B[L + 2]>(m, e.xd, ldsb, gtb);
+    if constexpr (L + 2 == LV::N) {
+      if (pred && m) m = apply_filters2<NL>(m, e.xd, ldsb);
+      emit_survivors2(pred ? m : 0, e.xd, *a, lane);
+    } else {
+      push(L + 1, e.xd, m, pred && m != 0);
+      drain<L + 1>();
+    }
+  }
+
+  template <int L>
+  __device__ __forceinline__ void drain() {
+    while (qn[L] >= 64) {
+      qn[L] -= 64;
+      const QEntry e = q[L][qn[L] + lane];
+      __builtin_amdgcn_wave_barrier();
+      run_level<L>(e, true);
+    }
+  }
+
+  template <int L>
+  __device__ __forceinline__ void flush() {
+    if constexpr (L < NQ) {
+      drain<L>();
+      const int n = qn[L];
+      if (n > 0) {
+        const bool pred = lane < n;
+        QEntry e{TWO52, 0};
+        if (pred) e = q[L][lane];
+        __builtin_amdgcn_wave_barrier();
+        qn[L] = 0;
+        run_level<L>(e, pred);
+      }
+      flush<L + 1>();
+    }
+  }
+};
+
+template <int NL, class LV>
+__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveArgs &a, int64_t tid,
+                                               int64_t stride) {
+  const double S59d = (double)a.kp.S59;
+  const double MODd = (double)AP26_MOD;
+  const double WRAP = MODd + TWO52;
+  int64_t b = a.blk0 + tid;
+  bool has = b < a.blk1;
+  double xd = has ? (double)n53_of_block(a.kp, a.n43, b) + TWO52 : TWO52;
+  int i59 = 0;
+  while (__ballot(has) != 0) {
+    uint64_t m0 = 0, m1 = 0;
+    if (has) level0_pair<0, LV::B[1]>(xd, qs.ldsb, m0, m1);
+    qs.push(0, xd, m0, has && m0 != 0);
+    qs.template drain<0>();
+    qs.push(0, xd + TWO48, m1, has && m1 != 0);
+    qs.template drain<0>();
+    if (has) {
+      xd += S59d;
+      if (xd >= WRAP) xd -= MODd;
+      if (++i59 == AP26_N59_STEPS) {
+        i59 = 0;
+        b += stride;
+        has = b < a.blk1;
+        if (has) xd = (double)n53_of_block(a.kp, a.n43, b) + TWO52;
+      }
+    }
+  }
+  qs.template flush<0>();
+}
+
+template <int NL, int WAVES, class LV>
+struct QSieve2Lds {
+  uint64_t tab[2 * (PT.off[NL] + 90)];
+  QEntry queues[WAVES][LV::N - 1][128];
+};
+
+template <int NL, int WAVES, class LV>
+constexpr int qsieve2_lds_bytes() {
+  return 2 * (PT.off[NL] + 90) * 8 + WAVES * (LV::N - 1) * 128 * (int)sizeof(QEntry);
+}
+
+/* Items (job, shift pair, n43 chunk); tables [entry][shift in pair]. */
+template <int NL, int WAVES, class LV>
+static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
+sieve_persistent2_kernel(PersistArgs p) {
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
+  constexpr int LMAIN2 = 2 * PT.off[NL];
+  constexpr int NPASS = AP26_NSHIFT / 2;
+  __shared__ QSieve2Lds<NL, WAVES, LV> L;
+  __shared__ unsigned int s_item;
+  const RoundState rs = *p.rs;
+  const int njobs = min(rs.per_round, rs.njobs - rs.base);
+  const unsigned int nitems = (unsigned int)(njobs > 0 ? njobs : 0) * NPASS * p.nchunk;
+  int cur = -1;
+  while (true) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
+    __syncthreads();
+    const unsigned int item = s_item;
+    if (item >= nitems) break;
+    const int jl = (int)(item / (NPASS * p.nchunk));
+    const int rem = (int)(item - (unsigned int)jl * NPASS * p.nchunk);
+    const int t = rem / p.nchunk;
+    const int chunk = rem - t * p.nchunk;
+    const BatchJobDev &job = p.jobs[rs.base + jl];
+    if (2 * t + 1 < job.it0) continue;
+    const uint64_t *tab = p.tab + ((size_t)jl * NPASS + t) * NTAB * 2;
+    const int key = jl * NPASS + t;
+    if (key != cur) {
+      for (int i = threadIdx.x; i < LMAIN2; i += blockDim.x) L.tab[i] = tab[i];
+      for (int i = threadIdx.x; i < 180; i += blockDim.x) L.tab[LMAIN2 + i] = tab[2 * NOK + i];
+      __syncthreads();
+      cur = key;
+    }
+    SieveArgs a;
+    a.kp = job.kp;
+    a.n43 = p.n43 + (size_t)jl * AP26_NN43;
+    a.tab = tab;
+    a.shift0 = job.kp.SHIFT + 128 * t;
+    a.nslot = 2;
+    const int j0 = chunk * p.chunk;
+    const int j1 = min(AP26_NN43, j0 + p.chunk);
+    a.blk0 = (int64_t)j0 * AP26_N53_PER_N43;
+    a.blk1 = (int64_t)j1 * AP26_N53_PER_N43;
+    a.surv = p.surv;
+    a.nsurv = p.nsurv;
+    a.cap = p.cap;
+    a.surv_job = p.surv_job;
+    a.job = jl;
+    QSieve2<NL, LV> qs;
+    qs.lane = threadIdx.x & 63;
+    qs.q = L.queues[threadIdx.x >> 6];
+    for (int i = 0; i < QSieve2<NL, LV>::NQ; i++) qs.qn[i] = 0;
+    qs.ldsb = (const char *)L.tab;
+    qs.gtb = (const char *)tab;
+    qs.a = &a;
+    qsieve2_blocks<NL, LV>(qs, a, threadIdx.x, blockDim.x);
+  }
 }
 
 struct BatchWalkArgs {

@@ -44,6 +44,10 @@ class Opts(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_int32), ("verbose", ctypes.c_int32)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int32), ("chunk", ctypes.c_int32), ("ms_per_k", ctypes.c_double)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("k_searched", ctypes.c_int64), ("k_skipped", ctypes.c_int64),
                 ("positions", ctypes.c_int64), ("survivors", ctypes.c_int64),
@@ -89,6 +93,8 @@ SIGNATURES = {
     "ap26_server_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                          ctypes.POINTER(_I64), ctypes.POINTER(_I32),
                                          ctypes.POINTER(_I32)]),
+    "ap26_autotune": (ctypes.c_int, [_P, _I32, _I32, _I32, ctypes.c_char_p, ctypes.POINTER(TuneEntry),
+                                     _I32, ctypes.POINTER(_I32), ctypes.POINTER(TuneEntry)]),
     "ap26_dev_alloc": (ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(_P)]),
     "ap26_dev_free": (ctypes.c_int, [_P, _P]),
     "ap26_copy_h2d": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
@@ -217,6 +223,17 @@ class Context:
                                          _ptr(tot), ctypes.byref(st))
         _check(rc, "ap26_search_batch")
         return out[:n.value].copy(), tot, st
+
+    def autotune(self, K=48620637, shift=640, quick=True, cache_path=None):
+        """Pick the batched sieve variant / work-item size by measurement."""
+        res = (TuneEntry * 64)()
+        n = ctypes.c_int32()
+        best = TuneEntry()
+        _check(self._lib.ap26_autotune(self.h, K, shift, 1 if quick else 0,
+                                       cache_path.encode() if cache_path else None, res, 64,
+                                       ctypes.byref(n), ctypes.byref(best)), "ap26_autotune")
+        return ({"variant": best.variant, "chunk": best.chunk, "ms_per_k": best.ms_per_k},
+                [{"variant": r.variant, "chunk": r.chunk, "ms_per_k": r.ms_per_k} for r in res[:n.value]])
 
     # device buffers
     def dev_alloc(self, nbytes):

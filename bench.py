@@ -13,6 +13,9 @@ K magnitudes.  The whole unit is searched -- tables, all 10 shifts of the
 sieve, the SPRP walk, AP check, per-range hash totals on the device -- and the
 host applies ReportSolution (9-base check, hash) to every report.
 
+The batched engine's sieve variant and work-item size are chosen first by the
+library's autotuner (ap26_autotune, 10 candidates timed on K=48620637).
+
 Beside it on the line:
   plain         the plain per-K path (one kernel sequence per K, host sync per
                 K) on consecutive K from 48620635 at SHIFT 640;
@@ -246,6 +249,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 24 candidates")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -253,6 +257,7 @@ def main():
         sys.exit(2)
 
     ctx = gpu.Context(0)
+    tune_best, tune_all = ctx.autotune(K=K0_C3, shift=SHIFT_C3, quick=not args.full_tune)
     b = run_batched(ctx, args.steps, args.warmup)
     kps = b["k"] / b["seconds"]
     st = b["stats"]
@@ -261,6 +266,9 @@ def main():
     avg_launch_ms = st["ms_sieve"] / rounds
     ops = pos_per_launch * LOOKUPS_PER_MASK_MIX * OPS_PER_LOOKUP
     achieved = ops / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else None
+    pair = tune_best["variant"] == 0 or tune_best["variant"] >= 10
+    kernel_name = ("sieve_persistent2_kernel" if pair else "sieve_persistent_kernel") + \
+        f" (batched variant {tune_best['variant']}, chunk {tune_best['chunk']})"
     line = {
         "metric": METRIC,
         "value": kps,
@@ -278,10 +286,12 @@ def main():
                                "workload (default_rng(26); len floor(exp(U(0,ln 4096))), start U[1e6,1e8], "
                                "SHIFT {0,640,1280}), one captured hipGraph per round of 16 K",
                    "k_searched": b["k"], "units": b["units"], "shifts": b["shifts"],
-                   "rounds": rounds, "path": "batched"},
+                   "rounds": rounds, "path": "batched",
+                   "autotuned": {"variant": tune_best["variant"], "chunk": tune_best["chunk"],
+                                 "candidates": len(tune_all)}},
         "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
                      "frac": achieved / PEAK_INT32_TOPS if achieved else None, "traffic": None,
-                     "kernel": "sieve_persistent_kernel", "avg_launch_ms": avg_launch_ms,
+                     "kernel": kernel_name, "avg_launch_ms": avg_launch_ms,
                      "ops_per_launch": ops,
                      "basis": f"{LOOKUPS_PER_MASK_MIX:.3f} lookups/mask x {OPS_PER_LOOKUP} int32 ops x "
                               f"{pos_per_launch:.4g} masks/launch (one launch = one round)"},

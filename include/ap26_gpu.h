@@ -152,6 +152,21 @@ int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out, int32_t ca
 int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int64_t *rounds,
                       int32_t *k_per_round, int32_t *active);
 
+/* ---- autotuner --------------------------------------------------------
+   Times the batched engine's sieve variants (LDS-staged primes, wavefronts
+   per CU, compaction levels, one shift or a shift pair per work item) and
+   work-item sizes on one K at SHIFT, keeps the fastest for this ctx, and
+   writes it to cache_path (if not NULL); ap26_open adopts a cache named by the
+   AP26_TUNE_CACHE environment variable.  quick = 1 tries 10 candidates. */
+typedef struct {
+  int32_t variant;
+  int32_t chunk;     /* n43 starts per work item */
+  double ms_per_k;   /* device time of the sample K */
+} ap26_tune_entry;
+
+int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,
+                  ap26_tune_entry *results, int32_t cap, int32_t *n_out, ap26_tune_entry *best);
+
 /* Device buffer helpers (so callers need no HIP/torch to stage inputs). */
 int ap26_dev_alloc(ap26_ctx *ctx, size_t bytes, void **dptr);
 int ap26_dev_free(ap26_ctx *ctx, void *dptr);

@@ -120,3 +120,20 @@ def test_server_admission_limit(gpu_ctx):
         assert srv.submit(366386, 366386, 0) is not None
         e = srv.submit(29, 29, 0)  # nothing to search: finished at once
         assert srv.result(e) is not None
+
+
+def test_autotune_picks_and_caches(tmp_path, monkeypatch):
+    cache = tmp_path / "tune.txt"
+    with gpu.Context(0) as ctx:
+        best, tried = ctx.autotune(K=875859, shift=640, quick=True, cache_path=str(cache))
+        assert len(tried) == 10
+        assert best["ms_per_k"] == min(t["ms_per_k"] for t in tried)
+        want = plain_reports(ctx, 875859, 875859, 640)
+        sols, tot, st = ctx.search_batch([(875859, 875859, 640, 0)])  # built with the choice
+        assert key(sols) == want
+    fields = cache.read_text().split()
+    assert fields[0] == "ap26-tune" and int(fields[3]) == best["variant"] and int(fields[4]) == best["chunk"]
+    monkeypatch.setenv("AP26_TUNE_CACHE", str(cache))
+    with gpu.Context(0) as ctx:  # adopts the cached choice
+        sols, tot, st = ctx.search_batch([(875859, 875859, 640, 0)])
+        assert key(sols) == want
