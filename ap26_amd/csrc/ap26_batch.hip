@@ -65,7 +65,8 @@ struct BatchState {
   int chunk = 0, nchunk = 0;
   int pvariant = -1;
   PersistVariant pv{};
-  int64_t surv_cap = 0;
+  int64_t surv_cap = 0;      /* survivor slots per round */
+  int64_t surv_slack = 0;    /* slots the sieve's waves may reserve and leave unused */
   int max_jobs = 0;          /* capacity of d_jobs */
   int max_ranges = 0;
   BatchJobDev *d_jobs = nullptr;
@@ -75,7 +76,7 @@ struct BatchState {
   uint64_t *d_tab = nullptr;
   int64_t *d_surv = nullptr;
   int32_t *d_surv_job = nullptr;
-  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) */
+  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) [4] overflow [5] survivors */
   DevSol *d_sols = nullptr;
   unsigned long long *d_hash = nullptr;
   /* one round = graph A (n43 starts, tables, persistent sieve) then graph B
@@ -123,7 +124,8 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
   e = getenv("AP26_BATCH_VARIANT");
   b->pvariant = e ? atoi(e) : (ctx->tune_variant >= 0 ? ctx->tune_variant : 0);
   b->pv = pick_pvariant(b->pvariant);
-  b->surv_cap = (int64_t)b->per_round * (INT64_C(60) << 20); /* ~45M survivors per K */
+  b->surv_slack = (int64_t)ctx->cu_count * b->pv.waves * SURV_CHUNK;
+  b->surv_cap = (int64_t)b->per_round * (INT64_C(48) << 20) + b->surv_slack; /* ~40M survivors per K */
   /* guarded (i31, i37, i41) triples in the reference order */
   std::vector<uint32_t> trip;
   for (int i31 = 0; i31 < 7; i31++)
@@ -186,6 +188,31 @@ static int ensure_capacity(BatchState *b, int njobs, int nranges) {
   return AP26_OK;
 }
 
+/* Grow the per-round survivor buffers to `need` slots; the graphs hold the
+   old pointers and are captured again. */
+static int ensure_surv_cap(ap26_ctx *ctx, BatchState *b, int64_t need) {
+  if (need <= b->surv_cap) return AP26_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(b->d_surv);
+  (void)hipFree(b->d_surv_job);
+  b->d_surv = nullptr;
+  b->d_surv_job = nullptr;
+  b->surv_cap = 0;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
+  b->graph = b->graph_b = nullptr;
+  if (hipMalloc(&b->d_surv, sizeof(int64_t) * need) != hipSuccess ||
+      hipMalloc(&b->d_surv_job, sizeof(int32_t) * need) != hipSuccess) {
+    (void)hipFree(b->d_surv);
+    (void)hipFree(b->d_surv_job);
+    b->d_surv = nullptr;
+    b->d_surv_job = nullptr;
+    return AP26_E_NOMEM;
+  }
+  b->surv_cap = need;
+  return AP26_OK;
+}
+
 /* enqueue one round on ctx->stream (captured into the graphs): part 0 =
    n43 starts, tables, sieve; part 1 = walk, advance */
 static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part) {
@@ -222,6 +249,8 @@ walk:
   w.sol_cap = BATCH_SOL_CAP;
   w.ntests = b->d_cnt + 1;
   w.hash_tot = b->d_hash;
+  w.overflow = b->d_cnt + 4;
+  w.nlive = b->d_cnt + 5;
   hipLaunchKernelGGL(batch_walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
   hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_rs, b->d_cnt + 0,
                      (unsigned int *)(b->d_cnt + 3));
@@ -276,7 +305,7 @@ static BatchJobDev make_job(int32_t K, int32_t shift, int32_t it0, int32_t range
 }
 
 static bool unit_ok(const ap26_range &u) {
-  return !(u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || u.shift < 0 || u.shift > (1 << 20) ||
+  return !(u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || !host_shift_ok(u.kmax, u.shift) ||
            u.iter_start < 0 || u.iter_start > AP26_NSHIFT);
 }
 
@@ -326,14 +355,15 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   BatchState *b = ctx->batch;
   rc = ensure_capacity(b, (int)jobs.size(), nr);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(BatchJobDev) * jobs.size(), hipMemcpyHostToDevice,
-                        ctx->stream));
-  const int nres = nr > 8 ? nr : 8;
-  hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs,
-                     (int)jobs.size(), b->per_round, b->d_cnt, b->d_hash, nr);
-  HIPCHK(hipGetLastError());
-  rc = ensure_graph(ctx, b);
-  if (rc) return rc;
+  /* survivor slots: the heaviest round's budget (K divisible by sieve
+     primes keep more survivors) */
+  int64_t need = 0;
+  for (size_t r0 = 0; r0 < jobs.size(); r0 += (size_t)b->per_round) {
+    int64_t sum = 0;
+    for (size_t j = r0; j < jobs.size() && j < r0 + (size_t)b->per_round; j++) sum += host_surv_budget(jobs[j].kp.K);
+    need = sum > need ? sum : need;
+  }
+  need += b->surv_slack;
   const int rounds = (int)((jobs.size() + b->per_round - 1) / b->per_round);
   const int timed = rounds < 1024 ? rounds : 1024;
   while ((int)b->ev.size() < 2 * timed) {
@@ -341,18 +371,35 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     HIPCHK(hipEventCreate(&e));
     b->ev.push_back(e);
   }
-  HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-  for (int i = 0; i < rounds; i++) {
-    rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
-    if (rc) return rc;
-  }
-  HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
   unsigned long long cnt[8];
-  HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   std::vector<unsigned long long> tot(nr);
-  HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
-                        ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int attempt = 0;; attempt++) {
+    rc = ensure_surv_cap(ctx, b, need);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(BatchJobDev) * jobs.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    const int nres = nr > 8 ? nr : 8;
+    hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs,
+                       (int)jobs.size(), b->per_round, b->d_cnt, b->d_hash, nr);
+    HIPCHK(hipGetLastError());
+    rc = ensure_graph(ctx, b);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+    for (int i = 0; i < rounds; i++) {
+      rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
+      if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
+    HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    /* a round kept more survivors than its budget: grow the buffers to the
+       largest count seen and run the batch again */
+    if (cnt[4] == 0) break;
+    if (attempt) return AP26_E_INTERNAL;
+    need = (int64_t)(cnt[4] + (cnt[4] >> 3)) + b->surv_slack;
+  }
   const unsigned int nsol = (unsigned int)(cnt[2] & 0xffffffffu);
   if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
   std::vector<DevSol> hs(nsol);
@@ -369,6 +416,7 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   if (hash_totals)
     for (int i = 0; i < nr; i++) hash_totals[i] = (int64_t)tot[i];
   st.prp_tests = (int64_t)cnt[1];
+  st.survivors = (int64_t)cnt[5];
   st.reports = nsol;
   for (const BatchJobDev &j : jobs)
     st.positions += (int64_t)AP26_NN43 * AP26_N53_PER_N43 * AP26_N59_STEPS * (AP26_NSHIFT - j.it0);
@@ -448,7 +496,7 @@ extern "C" int ap26_server_open(ap26_ctx *ctx, int32_t max_units, int64_t mem_bu
   s->ctx = ctx;
   s->max_units = max_units;
   int64_t budget = mem_budget_bytes > 0 ? mem_budget_bytes : (int64_t)(ctx->global_mem / 8);
-  const int64_t per_k = (int64_t)NTAB * AP26_NSHIFT * 8 + (int64_t)AP26_NN43 * 8 + (INT64_C(60) << 20) * 12;
+  const int64_t per_k = (int64_t)NTAB * AP26_NSHIFT * 8 + (int64_t)AP26_NN43 * 8 + (INT64_C(48) << 20) * 12;
   int R = (int)(budget / per_k);
   if (R < 1) {
     delete s;
@@ -521,17 +569,26 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   if (completed) *completed = 0;
   if (s->active.empty()) return AP26_OK;
   HIPCHK(hipSetDevice(ctx->device));
-  /* one K from each active unit in turn until the round is full */
+  /* one K from each active unit in turn until the round is full: R jobs, or
+     the survivor slots (a K divisible by sieve primes takes more of them;
+     one that needs more than the buffers hold alone grows them) */
   std::vector<int64_t> owner;
   int n = 0;
-  bool progress = true;
-  while (n < s->R && progress) {
+  int64_t slots = b->surv_slack;
+  bool progress = true, full = false;
+  while (n < s->R && progress && !full) {
     progress = false;
     for (size_t c = 0; c < s->active.size() && n < s->R; c++) {
       const size_t idx = (s->rr + c) % s->active.size();
       SUnit &u = s->units[s->active[idx]];
       if (u.next >= u.ks.size()) continue;
       const int32_t K = u.ks[u.next];
+      const int64_t need = host_surv_budget(K);
+      if (n > 0 && slots + need > b->surv_cap) {
+        full = true;
+        break;
+      }
+      slots += need;
       const int it0 = (u.next == 0 && K == u.r.kmin) ? u.r.iter_start : 0;
       u.next++;
       u.outstanding++;
@@ -546,9 +603,12 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   s->h_rs->njobs = n;
   s->h_rs->per_round = s->R;
   s->h_rs->pad = 0;
+  int rc = ensure_surv_cap(ctx, b, slots);
+  if (!rc) rc = ensure_graph(ctx, b);
+  if (rc) return rc;
   HIPCHK(hipMemcpyAsync(b->d_jobs, s->h_jobs, sizeof(BatchJobDev) * n, hipMemcpyHostToDevice, ctx->stream));
   HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
-  int rc = launch_round(ctx, b, nullptr, nullptr);
+  rc = launch_round(ctx, b, nullptr, nullptr);
   if (rc) return rc;
   HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
@@ -557,6 +617,7 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   s->rounds++;
   s->k_done += n;
   const unsigned int nsol = (unsigned int)(s->h_cnt[2] & 0xffffffffu);
+  if (s->h_cnt[4] != 0) return AP26_E_INTERNAL; /* survivors beyond the round's budget */
   if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
   std::vector<DevSol> hs(nsol);
   if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
@@ -640,7 +701,7 @@ static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,
                              ap26_tune_entry *results, int32_t cap, int32_t *n_out, ap26_tune_entry *best) {
-  if (!ctx || !host_will_search(K) || SHIFT < 0 || SHIFT > (1 << 20)) return AP26_E_RANGE;
+  if (!ctx || K < 1 || K > 600000000 || !host_will_search(K) || !host_shift_ok(K, SHIFT)) return AP26_E_RANGE;
   HIPCHK(hipSetDevice(ctx->device));
   const int nv = quick ? 5 : (int)(sizeof(TUNE_VARIANTS) / sizeof(int));
   const int nc = quick ? 2 : (int)(sizeof(TUNE_CHUNKS) / sizeof(int));

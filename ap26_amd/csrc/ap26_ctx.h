@@ -24,7 +24,7 @@ using namespace ap26;
   } while (0)
 
 
-constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* 67M survivors (~39M per K) */
+constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* initial survivor buffer: 67M (~40M per K) */
 constexpr int SOL_CAP = 1 << 16;
 constexpr int NCOUNT = 8; /* device counters */
 enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2 };
@@ -60,7 +60,8 @@ struct ap26_ctx {
   /* device buffers */
   uint64_t *d_tab = nullptr;     /* 10 shifts of tables */
   int64_t *d_n43 = nullptr;      /* [10840] */
-  int64_t *d_surv = nullptr;     /* SURV_CAP */
+  int64_t *d_surv = nullptr;     /* surv_cap entries, grown for K with more survivors */
+  int64_t surv_cap = 0;
   unsigned long long *d_cnt = nullptr; /* NCOUNT counters (sol count stored as u32 at C_NSOL) */
   DevSol *d_sols = nullptr;      /* SOL_CAP */
   SieveVariant sieve{};

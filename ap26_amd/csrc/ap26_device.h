@@ -118,26 +118,68 @@ __device__ __forceinline__ uint64_t dbl_mod(uint64_t x, uint64_t n) {
   return x >= t ? x - t : x + x;
 }
 
+/* Montgomery squaring for odd n < 2^63, times 2^s (s = 0 or 1):
+   returns 2^s * a^2 / 2^64 mod n for a < n, with ninv = n^-1 mod 2^64.
+   Since a < 2^63, a1 = a >> 32 < 2^31 and T = a^2 is three 32x32->64
+   multiply-adds with no carries: t = a0*(2*a1) + hi32(a0^2) < 2^64 and
+   hi(T) = a1^2 + hi32(t).  T * 2^s < 2 n^2, so hi(T * 2^s) < n when
+   n < 2^63.  Reduction: m = lo(T) * ninv, so m*n and T agree in the low
+   64 bits and (T - m*n) / 2^64 = hi(T) - hi(m*n) exactly, a value in
+   (-n, n) that one conditional add of n brings into [0, n). */
+__device__ __forceinline__ uint64_t mont_sqr63(uint64_t a, uint32_t s, uint64_t n,
+                                               uint64_t ninv) {
+  const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+  const uint64_t p0 = (uint64_t)a0 * a0;
+  const uint64_t t = (uint64_t)a0 * (a1 << 1) + (p0 >> 32);
+  uint64_t hi = (uint64_t)a1 * a1 + (t >> 32);
+  uint64_t lo = (t << 32) | (uint32_t)p0;
+  hi = (hi << s) | ((lo >> 63) & s);
+  lo <<= s;
+  const uint64_t m = lo * ninv;
+  const uint64_t h = __umul64hi(m, n);
+  const uint64_t u = hi - h;
+  return hi < h ? u + n : u;
+}
+
 /* Base-2 strong probable-prime test (textbook Miller-Rabin, base 2), the
  * predicate of PrimeQ (src/common/PrimeQ_x86.h:114-169): n odd >= 3 with
  * n-1 = d*2^t, d odd, passes iff 2^d == 1 or 2^(d*2^s) == -1 (mod n) for some
  * 0 <= s < t.  Valid for every odd n < 2^64.  Left-to-right exponentiation in
  * Montgomery form: one squaring per bit of d after the first, a modular
- * doubling for each set bit (base 2). */
+ * doubling for each set bit (base 2).  n < 2^63 (every search term and the
+ * micro-benchmark's range) takes the squaring above; larger n the general
+ * multiply with its carry handling. */
 __device__ __forceinline__ bool sprp2(uint64_t n) {
   if (n < 3 || !(n & 1)) return false;
-  Mont M{n, neg_inverse(n)};
+  const uint64_t ninv = neg_inverse(n);
   uint64_t d = n - 1;
-  int t = __builtin_ctzll(d);
+  const int t = __builtin_ctzll(d);
   d >>= t;
-  int bl = 64 - __builtin_clzll(n);
+  const int bl = 64 - __builtin_clzll(n);
   /* x = 2^65 mod n = Montgomery form of 2, by doubling 2^(bl-1) < n */
   uint64_t x = (uint64_t)1 << (bl - 1);
   for (int i = 0; i < 66 - bl; i++) x = dbl_mod(x, n);
   /* one = R mod n = x/2 mod n */
-  uint64_t one = (x & 1) ? (x >> 1) + (n >> 1) + 1 : (x >> 1);
-  uint64_t mone = n - one;
-  int L = 64 - __builtin_clzll(d);
+  const uint64_t one = (x & 1) ? (x >> 1) + (n >> 1) + 1 : (x >> 1);
+  const uint64_t mone = n - one;
+  const int L = 64 - __builtin_clzll(d);
+  if (n < (UINT64_C(1) << 63)) {
+    const uint64_t ninvp = (uint64_t)0 - ninv;
+    /* the bits of d below its leading one, from the top, at bit 63 */
+    uint64_t dd = L > 1 ? d << (65 - L) : 0;
+    for (int b = L - 2; b >= 0; b--) {
+      x = mont_sqr63(x, (uint32_t)(dd >> 63), n, ninvp);
+      dd <<= 1;
+    }
+    if (x == one || x == mone) return true;
+    for (int s = 1; s < t; s++) {
+      x = mont_sqr63(x, 0, n, ninvp);
+      if (x == mone) return true;
+      if (x == one) return false;
+    }
+    return false;
+  }
+  const Mont M{n, ninv};
   for (int b = L - 2; b >= 0; b--) {
     x = mont_mul(x, x, M);
     if ((d >> b) & 1) x = dbl_mod(x, n);

@@ -97,6 +97,7 @@ int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
       rc = AP26_E_NOMEM;
       break;
     }
+    ctx->surv_cap = SURV_CAP;
     ctx->sieve = pick_variant(ctx->opts.sieve_variant);
     tune_load_cache(ctx);
     ctx->opts.block = ctx->sieve.block;
@@ -171,7 +172,7 @@ static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int
     a.blk1 = (int64_t)j1 * AP26_N53_PER_N43;
     a.surv = ctx->d_surv;
     a.nsurv = ctx->d_cnt + C_NSURV;
-    a.cap = SURV_CAP;
+    a.cap = ctx->surv_cap;
     a.surv_job = nullptr;
     a.job = 0;
     hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
@@ -207,9 +208,26 @@ static int prepare_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, KParams *kp) {
   return AP26_OK;
 }
 
+/* Grow the survivor buffer to at least `need` entries (contents dropped). */
+static int ensure_surv(ap26_ctx *ctx, int64_t need) {
+  if (need <= ctx->surv_cap) return AP26_OK;
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(ctx->d_surv);
+  ctx->d_surv = nullptr;
+  ctx->surv_cap = 0;
+  if (hipMalloc(&ctx->d_surv, sizeof(int64_t) * need) != hipSuccess) {
+    ctx->d_surv = nullptr;
+    if (hipMalloc(&ctx->d_surv, sizeof(int64_t) * SURV_CAP) != hipSuccess) return AP26_E_NOMEM;
+    ctx->surv_cap = SURV_CAP;
+    return AP26_E_NOMEM;
+  }
+  ctx->surv_cap = need;
+  return AP26_OK;
+}
+
 static bool span_ok(int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1) {
   if (K < 1 || K > 600000000) return false;       /* per-K constants exact below ~6.3e8 */
-  if (SHIFT < 0 || SHIFT > 1 << 20) return false; /* (jj+SHIFT)*MOD stays below 2^63 */
+  if (!host_shift_ok(K, SHIFT)) return false;   /* every term fits in int64 */
   if (it0 < 0 || it1 > AP26_NSHIFT || it0 > it1) return false;
   if (j0 < 0 || j1 > AP26_NN43 || j0 > j1) return false;
   return true;
@@ -230,33 +248,43 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     return AP26_OK;
   }
   KParams kp;
-  int rc = prepare_k(ctx, K, SHIFT, &kp);
+  int rc = ensure_surv(ctx, host_surv_budget(K));
   if (rc) return rc;
-  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-  rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, &st);
-  if (rc) return rc;
-  WalkArgs w;
-  w.surv = ctx->d_surv;
-  w.nsurv = ctx->d_cnt + C_NSURV;
-  w.cap = SURV_CAP;
-  w.STEP = kp.STEP;
-  w.K = K;
-  w.SHIFT = SHIFT;
-  w.range_id = -1;
-  w.sols = ctx->d_sols;
-  w.nsol = (unsigned int *)(ctx->d_cnt + C_NSOL);
-  w.sol_cap = SOL_CAP;
-  w.ntests = ctx->d_cnt + C_NTESTS;
-  hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
-  HIPCHK(hipGetLastError());
-  st.walk_launches++;
-  HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
   unsigned long long cnt[NCOUNT];
-  HIPCHK(hipMemcpyAsync(cnt, ctx->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int attempt = 0;; attempt++) {
+    st = ap26_stats{};
+    rc = prepare_k(ctx, K, SHIFT, &kp);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, &st);
+    if (rc) return rc;
+    WalkArgs w;
+    w.surv = ctx->d_surv;
+    w.nsurv = ctx->d_cnt + C_NSURV;
+    w.cap = ctx->surv_cap;
+    w.STEP = kp.STEP;
+    w.K = K;
+    w.SHIFT = SHIFT;
+    w.range_id = -1;
+    w.sols = ctx->d_sols;
+    w.nsol = (unsigned int *)(ctx->d_cnt + C_NSOL);
+    w.sol_cap = SOL_CAP;
+    w.ntests = ctx->d_cnt + C_NTESTS;
+    hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+    HIPCHK(hipGetLastError());
+    st.walk_launches++;
+    HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+    HIPCHK(hipMemcpyAsync(cnt, ctx->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    /* more survivors than the buffer held (not seen for any K <= 6e8 within
+       the budget): grow it to the count seen and search the span again */
+    if (cnt[C_NSURV] <= (unsigned long long)ctx->surv_cap) break;
+    if (attempt) return AP26_E_INTERNAL;
+    rc = ensure_surv(ctx, (int64_t)(cnt[C_NSURV] + (cnt[C_NSURV] >> 3)));
+    if (rc) return rc;
+  }
   const unsigned long long nsurv = cnt[C_NSURV];
   const unsigned int nsol = (unsigned int)(cnt[C_NSOL] & 0xffffffffu);
-  if (nsurv > (unsigned long long)SURV_CAP) return AP26_E_INTERNAL;
   if (nsol > (unsigned int)SOL_CAP) return AP26_E_INTERNAL;
   std::vector<DevSol> hs(nsol);
   if (nsol) {
@@ -296,15 +324,22 @@ int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_
   *n_out = 0;
   if (it0 == it1 || j0 == j1) return AP26_OK;
   KParams kp;
-  int rc = prepare_k(ctx, K, SHIFT, &kp);
-  if (rc) return rc;
-  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-  rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, nullptr);
+  int rc = ensure_surv(ctx, host_surv_budget(K));
   if (rc) return rc;
   unsigned long long nsurv = 0;
-  HIPCHK(hipMemcpyAsync(&nsurv, ctx->d_cnt + C_NSURV, sizeof(nsurv), hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  if (nsurv > (unsigned long long)SURV_CAP) return AP26_E_INTERNAL;
+  for (int attempt = 0;; attempt++) {
+    rc = prepare_k(ctx, K, SHIFT, &kp);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+    rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(&nsurv, ctx->d_cnt + C_NSURV, sizeof(nsurv), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (nsurv <= (unsigned long long)ctx->surv_cap) break;
+    if (attempt) return AP26_E_INTERNAL;
+    rc = ensure_surv(ctx, (int64_t)(nsurv + (nsurv >> 3)));
+    if (rc) return rc;
+  }
   *n_out = (int64_t)nsurv;
   int64_t ncopy = (int64_t)nsurv < cap ? (int64_t)nsurv : cap;
   if (ncopy > 0) HIPCHK(hipMemcpy(out, ctx->d_surv, sizeof(int64_t) * (size_t)ncopy, hipMemcpyDeviceToHost));

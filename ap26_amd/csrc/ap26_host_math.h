@@ -53,4 +53,33 @@ static inline void host_n43(const HostK *hk, int64_t *out) {
     }
 }
 
+/* Every term the search forms for K and SHIFT fits in int64: survivors are
+   below (SHIFT + 640) * MOD, and an AP with difference STEP holds at most 29
+   primes in a row (29 does not divide a searched K, so one term in 29 is a
+   multiple of 29), so the walk never passes survivor + 34 * STEP. */
+static inline int host_shift_ok(int32_t K, int32_t SHIFT) {
+  if (K < 1 || SHIFT < 0) return 0;
+  const __int128 top = ((__int128)SHIFT + 640) * AP26_MOD + (__int128)35 * K * AP26_PRIM23;
+  return top < ((__int128)1 << 63);
+}
+
+/* Survivor-buffer entries to hold for one K.  A sieve prime p (61..541)
+   rejects 24 of its p residues, or only residue 0 when p divides K (then
+   STEP = 0 mod p), so each such prime scales the expected survivor count by
+   (p - 1) / (p - 24): 1.62 for 61, and 5.45 for K = 61*67*71*73, the most a
+   K <= 6e8 can hold.  An ordinary K keeps 38-41 M survivors over its 10
+   shifts; the budget is 48 Mi entries times that factor. */
+static inline int64_t host_surv_budget(int32_t K) {
+  static const int sp[] = {61,  67,  71,  73,  79,  83,  89,  97,  101, 103, 107, 109, 113, 127,
+                           131, 137, 139, 149, 151, 157, 163, 167, 173, 179, 181, 191, 193, 197,
+                           199, 211, 223, 227, 229, 233, 239, 241, 251, 257, 263, 269, 271, 277,
+                           281, 283, 293, 307, 311, 313, 317, 331, 337, 347, 349, 353, 359, 367,
+                           373, 379, 383, 389, 397, 401, 409, 419, 421, 431, 433, 439, 443, 449,
+                           457, 461, 463, 467, 479, 487, 491, 499, 503, 509, 521, 523, 541};
+  double w = 48.0 * (1 << 20);
+  for (unsigned i = 0; i < sizeof(sp) / sizeof(sp[0]); i++)
+    if (K % sp[i] == 0) w *= (double)(sp[i] - 1) / (double)(sp[i] - 24);
+  return (int64_t)w;
+}
+
 #endif

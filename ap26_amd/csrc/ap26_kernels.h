@@ -705,7 +705,19 @@ __device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_
   m1 = ((uint64_t)a3 << 32) | a2;
 }
 
-__device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const SieveArgs &a, int lane) {
+/* Survivor slots of one wave in the batched sieve: the wave reserves
+   SURV_CHUNK consecutive slots with one atomic and fills them before taking
+   more, so the survivor list is written in whole cache lines and the global
+   counter sees one atomic per SURV_CHUNK survivors.  Slots a wave leaves
+   unused when it finishes hold -1, which the walk skips. */
+constexpr int SURV_CHUNK = 128;
+struct SurvCursor {
+  unsigned long long base; /* next free slot */
+  int left;                /* free slots left at base */
+};
+
+__device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const SieveArgs &a, int lane,
+                                                SurvCursor &cur) {
   const uint64_t bits = (uint64_t)__double_as_longlong(xd);
   const int e = (int)((bits >> 48) & 1);
   const int64_t n59 = (int64_t)(bits & ((INT64_C(1) << 48) - 1));
@@ -717,18 +729,36 @@ __device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const Sie
     const int cnt = __popcll(bal);
     const int leader = __ffsll((unsigned long long)bal) - 1;
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(a.nsurv, (unsigned long long)cnt);
-    base = __shfl(base, leader);
+    unsigned long long fresh = 0;
+    if (cnt > cur.left) { /* wave-uniform */
+      if (lane == leader) fresh = atomicAdd(a.nsurv, (unsigned long long)SURV_CHUNK);
+      fresh = __shfl(fresh, leader);
+    }
     if (has) {
       const int bit = __ffsll((unsigned long long)w) - 1;
       w &= w - 1;
-      const unsigned long long idx = base + (unsigned long long)pre;
+      const unsigned long long idx =
+          pre < cur.left ? cur.base + (unsigned long long)pre : fresh + (unsigned long long)(pre - cur.left);
       if (idx < (unsigned long long)a.cap) {
         a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
-        if (a.surv_job) a.surv_job[idx] = a.job;
+        a.surv_job[idx] = a.job;
       }
     }
+    if (cnt > cur.left) {
+      cur.base = fresh + (unsigned long long)(cnt - cur.left);
+      cur.left = SURV_CHUNK - (cnt - cur.left);
+    } else {
+      cur.base += (unsigned long long)cnt;
+      cur.left -= cnt;
+    }
+  }
+}
+
+/* Mark the wave's unused reserved slots as empty (-1). */
+__device__ __forceinline__ void close_survivors2(const SurvCursor &cur, int64_t *surv, int64_t cap, int lane) {
+  for (int i = lane; i < cur.left; i += 64) {
+    const unsigned long long idx = cur.base + (unsigned long long)i;
+    if (idx < (unsigned long long)cap) surv[idx] = -1;
   }
 }
 
@@ -741,6 +771,7 @@ struct QSieve2 {
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
+  SurvCursor *cur;
 
   __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
@@ -756,7 +787,7 @@ struct QSieve2 {
     if (pred) m = apply_primes2<NL, LV::B[L + 1], LV::B[L + 2]>(m, e.xd, ldsb, gtb);
     if constexpr (L + 2 == LV::N) {
       if (pred && m) m = apply_filters2<NL>(m, e.xd, ldsb);
-      emit_survivors2(pred ? m : 0, e.xd, *a, lane);
+      emit_survivors2(pred ? m : 0, e.xd, *a, lane, *cur);
     } else {
       push(L + 1, e.xd, m, pred && m != 0);
       drain<L + 1>();
@@ -847,12 +878,18 @@ sieve_persistent2_kernel(PersistArgs p) {
   const int njobs = min(rs.per_round, rs.njobs - rs.base);
   const unsigned int nitems = (unsigned int)(njobs > 0 ? njobs : 0) * NPASS * p.nchunk;
   int cur = -1;
+  __shared__ SurvCursor s_cur[WAVES]; /* per wave, in LDS to spare registers */
+  SurvCursor &sc = s_cur[threadIdx.x >> 6];
+  sc = SurvCursor{0, 0};
   while (true) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
     __syncthreads();
     const unsigned int item = s_item;
-    if (item >= nitems) break;
+    if (item >= nitems) {
+      close_survivors2(sc, p.surv, p.cap, threadIdx.x & 63);
+      break;
+    }
     const int jl = (int)(item / (NPASS * p.nchunk));
     const int rem = (int)(item - (unsigned int)jl * NPASS * p.nchunk);
     const int t = rem / p.nchunk;
@@ -889,6 +926,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
+    qs.cur = &sc;
     qsieve2_blocks<NL, LV>(qs, a, threadIdx.x, blockDim.x);
   }
 }
@@ -905,6 +943,8 @@ struct BatchWalkArgs {
   int32_t sol_cap;
   unsigned long long *__restrict__ ntests;
   unsigned long long *__restrict__ hash_tot; /* per range: exact sum of first%1000 + k */
+  unsigned long long *__restrict__ overflow; /* largest survivor count beyond cap */
+  unsigned long long *__restrict__ nlive;    /* survivors walked, over all rounds */
 };
 
 __device__ __forceinline__ int walk_one(int64_t c, int64_t STEP, int64_t *first, unsigned long long &tests) {
@@ -932,11 +972,16 @@ __device__ __forceinline__ int walk_one(int64_t c, int64_t STEP, int64_t *first,
 static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a) {
   const RoundState rs = *a.rs;
   unsigned long long n = *a.nsurv;
-  if (n > (unsigned long long)a.cap) n = (unsigned long long)a.cap;
-  unsigned long long tests = 0;
+  if (n > (unsigned long long)a.cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(a.overflow, n);
+    n = (unsigned long long)a.cap;
+  }
+  unsigned long long tests = 0, live = 0;
   for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (unsigned long long)gridDim.x * blockDim.x) {
     const int64_t c = a.surv[i];
+    if (c < 0) continue; /* slot reserved but unused */
+    live++;
     const BatchJobDev &job = a.jobs[rs.base + a.surv_job[i]];
     int64_t first;
     const int k = walk_one(c, job.kp.STEP, &first, tests);
@@ -955,6 +1000,7 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
     }
   }
   if (tests) atomicAdd(a.ntests, tests);
+  if (live) atomicAdd(a.nlive, live);
 }
 
 /* End of round: clear the per-round counters, advance to the next round. */

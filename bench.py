@@ -65,6 +65,20 @@ OPS_PER_LOOKUP = 4
 # 64-bit Montgomery squaring: 18 MUL32 + 16 add-class INT32 ops.
 OPS_PER_SQUARING = 34
 PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T int32 lane-ops/s (nominal)
+PEAK_HBM_GBS = 8000.0
+# HBM bytes per searched K moved by the persistent sieve kernel, from two
+# rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
+# workload (profiles/sieve_persistent2_traffic.md): FETCH_SIZE 226,100 KiB and
+# WRITE_SIZE 7,921,454 KiB per launch.  Both are scaled by what the same
+# counters read for prp_kernel over 2^26 values, which reads 512 MiB and
+# writes 64 MiB exactly: FETCH_SIZE 262,167 KiB (x2.000), WRITE_SIZE
+# 65,536 KiB (x1.000).  (2 x 226,100 + 7,921,454) KiB / 16 K:
+TRAFFIC_BYTES_PER_K = (2.0 * 226100.3 + 7921453.9) * 1024 / 16
+# Algorithmic bytes of the sieve: each survivor written once (8-byte value +
+# 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
+# n43 starts (10,840 x 8 B) read once.
+SURVIVOR_BYTES = 12
+TABLE_BYTES_PER_K = 10 * 23783 * 8 + 10840 * 8
 
 
 def will_search(K):
@@ -111,8 +125,9 @@ def report_all(hosts, sols):
 
 
 def run_batched(ctx, steps, warmup):
-    wu, _ = c4_sample(warmup, per_unit=1)
-    ctx.search_batch(wu)
+    if warmup > 0:
+        wu, _ = c4_sample(warmup, per_unit=1)
+        ctx.search_batch(wu)
     units, nk = c4_sample(steps)
     tmp = tempfile.mkdtemp(prefix="ap26bench")
     hosts = []
@@ -250,6 +265,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
     ap.add_argument("--full-tune", action="store_true", help="autotune over all 24 candidates")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (10, 85)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -257,7 +274,12 @@ def main():
         sys.exit(2)
 
     ctx = gpu.Context(0)
-    tune_best, tune_all = ctx.autotune(K=K0_C3, shift=SHIFT_C3, quick=not args.full_tune)
+    if args.no_tune:
+        tune_best = {"variant": int(os.environ.get("AP26_BATCH_VARIANT", "10")),
+                     "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "85"))}
+        tune_all = []
+    else:
+        tune_best, tune_all = ctx.autotune(K=K0_C3, shift=SHIFT_C3, quick=not args.full_tune)
     b = run_batched(ctx, args.steps, args.warmup)
     kps = b["k"] / b["seconds"]
     st = b["stats"]
@@ -266,6 +288,8 @@ def main():
     avg_launch_ms = st["ms_sieve"] / rounds
     ops = pos_per_launch * LOOKUPS_PER_MASK_MIX * OPS_PER_LOOKUP
     achieved = ops / (avg_launch_ms / 1e3) / 1e12 if avg_launch_ms > 0 else None
+    k_per_launch = b["k"] / rounds
+    alg_bytes = (st["survivors"] * SURVIVOR_BYTES + b["k"] * TABLE_BYTES_PER_K) / rounds
     pair = tune_best["variant"] == 0 or tune_best["variant"] >= 10
     kernel_name = ("sieve_persistent2_kernel" if pair else "sieve_persistent_kernel") + \
         f" (batched variant {tune_best['variant']}, chunk {tune_best['chunk']})"
@@ -290,7 +314,11 @@ def main():
                    "autotuned": {"variant": tune_best["variant"], "chunk": tune_best["chunk"],
                                  "candidates": len(tune_all)}},
         "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
-                     "frac": achieved / PEAK_INT32_TOPS if achieved else None, "traffic": None,
+                     "frac": achieved / PEAK_INT32_TOPS if achieved else None,
+                     "traffic": TRAFFIC_BYTES_PER_K * k_per_launch,
+                     "hbm": {"algorithmic_bytes": alg_bytes, "traffic_bytes": TRAFFIC_BYTES_PER_K * k_per_launch,
+                             "achieved_GBs": alg_bytes / (avg_launch_ms / 1e3) / 1e9,
+                             "frac_of_8TBs": alg_bytes / (avg_launch_ms / 1e3) / 1e9 / PEAK_HBM_GBS},
                      "kernel": kernel_name, "avg_launch_ms": avg_launch_ms,
                      "ops_per_launch": ops,
                      "basis": f"{LOOKUPS_PER_MASK_MIX:.3f} lookups/mask x {OPS_PER_LOOKUP} int32 ops x "

@@ -31,7 +31,9 @@ DST = os.path.join(HERE, "ref_min10")
 # SHIFT 640; c3_*: K from the 4096-K range starting at
 # default_rng(26).integers(40000000, 50000001) = 48620635; s1280_*: SHIFT 1280
 # at both ends of the batched K range; p61_*: a K divisible by 61, where
-# STEP = 0 (mod 61); range_*: a multi-K unit with a skipped K.
+# STEP = 0 (mod 61); h4_*: K = 61*67*71*73, the most sieve primes a searched
+# K <= 6e8 can hold, so the sieve keeps about 5.4x the usual survivors;
+# range_*: a multi-K unit with a skipped K.
 RUN_LIST = [
     ("t366384", 366384, 366384, 0),
     ("t44121555", 44121555, 44121555, 0),
@@ -44,6 +46,7 @@ RUN_LIST = [
     ("s1280_99999989", 99999989, 99999989, 1280),
     ("p61_976061", 976061, 976061, 0),
     ("range_875858_875860", 875858, 875860, 640),
+    ("h4_21182921", 21182921, 21182921, 0),
 ]
 
 

@@ -68,6 +68,8 @@ def test_batch_edge_cases(gpu_ctx):
     assert len(sols) == 0 and tot[0] == 0
     with pytest.raises(gpu.Ap26Error):
         gpu_ctx.search_batch([(5, 4, 0, 0), (0, 3, 0, 0)])
+    with pytest.raises(gpu.Ap26Error):  # terms past 2^63
+        gpu_ctx.search_batch([(366384, 366384, 36000, 0)])
 
 
 def test_server_alone_and_in_mix(gpu_ctx):
@@ -83,7 +85,7 @@ def test_server_alone_and_in_mix(gpu_ctx):
                 r = srv.result(t)
             alone.append((key(r[0]), r[1]))
     with gpu.Server(gpu_ctx, max_units=8, mem_budget_bytes=3 << 30) as srv:
-        assert srv.stats()["k_per_round"] == 4  # 3 GiB / ~0.76 GB per in-flight K
+        assert srv.stats()["k_per_round"] == 5  # 3 GiB / ~0.61 GB per in-flight K
         tickets = [srv.submit(*u) for u in units[:3]]
         srv.pump()                      # the first round runs with three units
         tickets += [srv.submit(*u) for u in units[3:]]  # late arrivals join
@@ -103,6 +105,29 @@ def test_server_alone_and_in_mix(gpu_ctx):
     # the plain path agrees
     for i, u in enumerate(units):
         assert alone[i][0] == plain_reports(gpu_ctx, *u)
+
+
+def test_server_heavy_k(gpu_ctx):
+    """A K with 5.4x the usual survivors (61*67*71*73 divides it) fills more
+    than a 1 GiB server's round buffers on its own: the server runs it in a
+    round of its own, grown to fit, and the unit beside it still matches."""
+    K = 61 * 67 * 71 * 73
+    units = [(K, K, 0), (875859, 875860, 640)]
+    with gpu.Server(gpu_ctx, max_units=4, mem_budget_bytes=1 << 30) as srv:
+        assert srv.stats()["k_per_round"] == 1
+        tickets = [srv.submit(*u) for u in units]
+        res = {}
+        while len(res) < len(units):
+            srv.pump()
+            for i, t in enumerate(tickets):
+                if i not in res:
+                    r = srv.result(t)
+                    if r is not None:
+                        res[i] = r
+    for i, (a, b, sh) in enumerate(units):
+        want = plain_reports(gpu_ctx, a, b, sh)
+        assert key(res[i][0]) == want, i
+        assert res[i][1] == sum(f % 1000 + k for k, _, f in want)
 
 
 def test_server_admission_limit(gpu_ctx):

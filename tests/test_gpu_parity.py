@@ -48,7 +48,8 @@ def test_prp_microbench_distribution_sample(gpu_ctx):
 
 SPANS = [(366384, 0, 0, 10, 0, 3), (875859, 640, 0, 10, 100, 102), (48620635, 640, 3, 7, 5000, 5002),
          (44121555, 0, 0, 10, 10838, 10840), (976061, 0, 2, 3, 7000, 7004),
-         (99999989, 1280, 9, 10, 1, 2), (1000003, 1280, 0, 10, 5419, 5421)]
+         (99999989, 1280, 9, 10, 1, 2), (1000003, 1280, 0, 10, 5419, 5421),
+         (61 * 67 * 71 * 73, 0, 0, 10, 3000, 3002)]
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2])
@@ -61,7 +62,7 @@ def test_sieve_span_parity(variant, span):
 
 
 @pytest.mark.parametrize("span", [(366384, 0, 0, 10, 0, 20), (875862, 640, 0, 10, 2000, 2020),
-                                  (44121555, 0, 0, 10, 10800, 10840)])
+                                  (44121555, 0, 0, 10, 10800, 10840), (61 * 67 * 71 * 73, 0, 0, 10, 100, 110)])
 def test_search_span_parity(gpu_ctx, span):
     g, st = gpu_ctx.search_span(*span)
     o, nsurv = O.search_span(*span, threads=4)
@@ -122,6 +123,25 @@ def test_full_k_survivor_invariants(gpu_ctx):
             assert np.all((r + i * sp) % p != 0)
 
 
+def test_heavy_k_grows_survivor_buffer(gpu_ctx):
+    """K = 61*67*71*73: four sieve primes divide STEP, so each rejects one
+    residue instead of 24 and the sieve keeps about 5.4x the usual survivors,
+    more than the initial 2^26-entry buffer.  The plain path grows its buffer,
+    the batched path budgets the round for it, and both agree."""
+    K = 61 * 67 * 71 * 73
+    sols, st = gpu_ctx.search_k(K, 0)
+    assert st.survivors > 1 << 26
+    want = sorted((int(a["k"]), int(a["first"])) for a in sols)
+    bs, tot, bst = gpu_ctx.search_batch([(366384, 366384, 0, 0), (K, K, 0, 0), (875859, 875859, 640, 0)])
+    got = sorted((int(a["k"]), int(a["first"])) for a in bs if a["range_id"] == 1)
+    assert got == want
+    assert tot[1] == sum(f % 1000 + k for k, f in want)
+    _, st2 = gpu_ctx.search_k(366384, 0)  # the grown buffer serves ordinary K as before
+    assert st2.survivors == 38838420
+    _, st3 = gpu_ctx.search_k(875859, 640)
+    assert bst.survivors == st.survivors + st2.survivors + st3.survivors
+
+
 def test_iter_start_is_a_suffix(gpu_ctx):
     full, _ = gpu_ctx.search_k(875862, 640)
     part, _ = gpu_ctx.search_k(875862, 640, iter_start=4)
@@ -137,7 +157,9 @@ def test_edge_cases(gpu_ctx):
     lib = gpu.load()
     n = ctypes.c_int32()
     for args in [(0, 0, 0, 10, 0, 1), (366384, 0, 0, 11, 0, 1), (366384, 0, 0, 10, 0, 10841),
-                 (366384, -64, 0, 10, 0, 1), (366384, 0, 5, 4, 0, 1)]:
+                 (366384, -64, 0, 10, 0, 1), (366384, 0, 5, 4, 0, 1),
+                 (366384, 36000, 0, 10, 0, 1), (366384, 1 << 20, 0, 10, 0, 1),
+                 (600000000, 20000, 0, 10, 0, 1)]:  # terms past 2^63
         rc = lib.ap26_search_span(gpu_ctx.h, *args, None, 0, ctypes.byref(n), None)
         assert rc == gpu.AP26_E_RANGE
     # capacity overflow reports the true count
