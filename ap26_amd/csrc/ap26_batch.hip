@@ -54,6 +54,13 @@ PersistVariant pick_pvariant(int id) {
     case 13: return make_pvariant2<20, 12, Levels5>();
     case 14: return make_pvariant2<31, 12, Levels4>();
     case 9: return make_pvariant<38, 16, Levels4>();
+    case 15: return make_pvariant2<20, 16, Levels4>();
+    case 16: return make_pvariant2<24, 16, Levels4>();
+    case 17: return make_pvariant2<18, 16, Levels4>();
+    case 18: return make_pvariant2<22, 16, Levels4>();
+    case 19: return make_pvariant2<15, 16, Levels4>();
+    case 20: return make_pvariant2<18, 16, Levels<0, 15, 22, 29, 45, 83>>();
+    case 21: return make_pvariant2<15, 16, Levels<0, 15, 22, 29, 45, 83>>();
     default: return make_pvariant2<29, 16, Levels4>();
   }
 }
