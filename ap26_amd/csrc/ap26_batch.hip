@@ -83,7 +83,7 @@ struct BatchState {
   uint64_t *d_tab = nullptr;
   int64_t *d_surv = nullptr;
   int32_t *d_surv_job = nullptr;
-  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) [4] overflow [5] survivors */
+  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) [4] overflow [5] survivors [6] err */
   DevSol *d_sols = nullptr;
   unsigned long long *d_hash = nullptr;
   /* one round = graph A (n43 starts, tables, persistent sieve) then graph B
@@ -239,6 +239,7 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part) {
   p.nsurv = b->d_cnt + 0;
   p.cap = b->surv_cap;
   p.work = (unsigned int *)(b->d_cnt + 3);
+  p.err = (unsigned int *)(b->d_cnt + 6);
   p.chunk = b->chunk;
   p.nchunk = b->nchunk;
   hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
@@ -401,6 +402,7 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
                           ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (cnt[6]) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
     /* a round kept more survivors than its budget: grow the buffers to the
        largest count seen and run the batch again */
     if (cnt[4] == 0) break;
@@ -439,6 +441,44 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
+extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, int64_t *out,
+                                  int64_t cap, int64_t *n_out) {
+  if (!ctx || !n_out || cap < 0 || (cap > 0 && !out)) return AP26_E_RANGE;
+  *n_out = 0;
+  const ap26_range u{K, K, SHIFT, iter_start};
+  if (!unit_ok(u) || K < 1 || !host_will_search(K)) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  int rc = batch_init(ctx);
+  if (rc) return rc;
+  BatchState *b = ctx->batch;
+  rc = ensure_capacity(b, 1, 1);
+  if (!rc) rc = ensure_surv_cap(ctx, b, host_surv_budget(K) + b->surv_slack);
+  if (rc) return rc;
+  const BatchJobDev job = make_job(K, SHIFT, iter_start, 0);
+  HIPCHK(hipMemcpy(b->d_jobs, &job, sizeof(job), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, b->d_rs, 1, b->per_round, b->d_cnt,
+                     b->d_hash, 1);
+  HIPCHK(hipGetLastError());
+  rc = ensure_graph(ctx, b);
+  if (rc) return rc;
+  HIPCHK(hipGraphLaunch(b->graph, ctx->stream)); /* n43 starts, tables, sieve: no walk */
+  unsigned long long cnt[8];
+  HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  const unsigned long long nres = cnt[0];
+  if (cnt[6] || nres > (unsigned long long)b->surv_cap) return AP26_E_INTERNAL;
+  std::vector<int64_t> all(nres);
+  if (nres) HIPCHK(hipMemcpy(all.data(), b->d_surv, sizeof(int64_t) * nres, hipMemcpyDeviceToHost));
+  int64_t n = 0;
+  for (int64_t v : all)
+    if (v >= 0) {
+      if (n < cap) out[n] = v;
+      n++;
+    }
+  *n_out = n;
+  return n > cap ? AP26_E_OVERFLOW : AP26_OK;
 }
 
 /* ---- serving ------------------------------------------------------------ */
@@ -625,6 +665,7 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   s->k_done += n;
   const unsigned int nsol = (unsigned int)(s->h_cnt[2] & 0xffffffffu);
   if (s->h_cnt[4] != 0) return AP26_E_INTERNAL; /* survivors beyond the round's budget */
+  if (s->h_cnt[6] != 0) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
   if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
   std::vector<DevSol> hs(nsol);
   if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));

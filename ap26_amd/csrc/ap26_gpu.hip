@@ -175,6 +175,7 @@ static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int
     a.cap = ctx->surv_cap;
     a.surv_job = nullptr;
     a.job = 0;
+    a.err = (unsigned int *)(ctx->d_cnt + C_ERR);
     hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
@@ -278,6 +279,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     HIPCHK(hipStreamSynchronize(ctx->stream));
     /* more survivors than the buffer held (not seen for any K <= 6e8 within
        the budget): grow it to the count seen and search the span again */
+    if (cnt[C_ERR]) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
     if (cnt[C_NSURV] <= (unsigned long long)ctx->surv_cap) break;
     if (attempt) return AP26_E_INTERNAL;
     rc = ensure_surv(ctx, (int64_t)(cnt[C_NSURV] + (cnt[C_NSURV] >> 3)));
@@ -333,8 +335,11 @@ int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
     rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, nullptr);
     if (rc) return rc;
+    unsigned long long err = 0;
     HIPCHK(hipMemcpyAsync(&nsurv, ctx->d_cnt + C_NSURV, sizeof(nsurv), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(&err, ctx->d_cnt + C_ERR, sizeof(err), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (err) return AP26_E_INTERNAL;
     if (nsurv <= (unsigned long long)ctx->surv_cap) break;
     if (attempt) return AP26_E_INTERNAL;
     rc = ensure_surv(ctx, (int64_t)(nsurv + (nsurv >> 3)));

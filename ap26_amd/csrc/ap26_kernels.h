@@ -103,6 +103,7 @@ struct SieveArgs {
   int64_t cap;
   int32_t *__restrict__ surv_job; /* batched engine: job index of each survivor */
   int32_t job;
+  unsigned int *__restrict__ err;   /* set if a queue step ran on part of a wave */
 };
 
 /* n53 = (n43[j] + i43*S43 + i47*S47 + i53*S53) mod MOD for block
@@ -360,6 +361,7 @@ struct QSieve {
   static constexpr int NQ = LV::N - 1;
   QEntry (*q)[QCAP];  /* this wave's NQ queues */
   int qn[NQ];
+  bool partial; /* a push ran with lanes switched off (must never happen) */
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
@@ -368,7 +370,10 @@ struct QSieve {
     const uint64_t bal = __ballot(pred);
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
     if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
-    qn[qi] += __popcll(bal);
+    /* the count is wave-uniform: keep it scalar, so that the queue loops
+       are uniform branches and every lane sees the same count */
+    qn[qi] = __builtin_amdgcn_readfirstlane(qn[qi] + __popcll(bal));
+    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -412,6 +417,8 @@ struct QSieve {
         run_level<L>(e, pred);
       }
       flush<L + 1>();
+    } else {
+      if (partial && lane == 0) atomicOr(a->err, 1u);
     }
   }
 };
@@ -429,30 +436,26 @@ __device__ __forceinline__ void qsieve_blocks(QSieve<NL, LV> &qs, const VConsts<
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
-  int64_t b = a.blk0 + tid;
-  bool has = b < a.blk1;
-  double xd = has ? (double)n53_of_block(a.kp, a.n43, b) + TWO52 : TWO52;
-  int i59 = 0;
-  while (__ballot(has) != 0) {
-    uint64_t m = 0;
-    if (has) {
+  /* Every lane runs every step of every stride that holds a block for any
+     lane of the wave: the queues need the whole wave, so the loops are
+     uniform and lanes past the end work on a valid block and drop it. */
+  for (int64_t base = a.blk0; base < a.blk1; base += stride) {
+    const int64_t b = base + tid;
+    const bool has = b < a.blk1;
+    if (__ballot(has) == 0) break;
+    double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
+    for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
+      uint64_t m;
       if constexpr (VC)
         m = apply_primes_vc<0, LV::B[1]>(~(uint64_t)0, xd, qs.ldsb, vc);
       else
         m = apply_primes<NL, 0, LV::B[1], G>(~(uint64_t)0, xd, qs.ldsb, qs.gtb);
-    }
-    qs.push(0, xd, m, has && m != 0);
-    if (has) {
+      m = has ? m : 0;
+      qs.push(0, xd, m, m != 0);
       xd += S59d;
-      if (xd >= WRAP) xd -= MODd;
-      if (++i59 == AP26_N59_STEPS) {
-        i59 = 0;
-        b += stride;
-        has = b < a.blk1;
-        if (has) xd = (double)n53_of_block(a.kp, a.n43, b) + TWO52;
-      }
+      xd = xd >= WRAP ? xd - MODd : xd;
+      qs.template drain<0>();
     }
-    qs.template drain<0>();
   }
   qs.template flush<0>();
 }
@@ -481,6 +484,7 @@ sieve_queue_kernel(SieveArgs a) {
   qs.lane = threadIdx.x & 63;
   qs.q = L.queues[threadIdx.x >> 6];
   for (int i = 0; i < QSieve<NL, LV>::NQ; i++) qs.qn[i] = 0;
+  qs.partial = false;
   qs.ldsb = (const char *)L.tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
@@ -515,6 +519,7 @@ struct PersistArgs {
   unsigned long long *__restrict__ nsurv;
   int64_t cap;
   unsigned int *__restrict__ work;   /* work-stealing item counter */
+  unsigned int *__restrict__ err;    /* queue steps on part of a wave */
   int32_t chunk;                     /* n43 starts per item */
   int32_t nchunk;
 };
@@ -569,11 +574,13 @@ sieve_persistent_kernel(PersistArgs p) {
     a.nsurv = p.nsurv;
     a.cap = p.cap;
     a.surv_job = p.surv_job;
+    a.err = p.err;
     a.job = jl;
     QSieve<NL, LV> qs;
     qs.lane = threadIdx.x & 63;
     qs.q = L.queues[threadIdx.x >> 6];
     for (int i = 0; i < QSieve<NL, LV>::NQ; i++) qs.qn[i] = 0;
+    qs.partial = false;
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
@@ -768,6 +775,7 @@ struct QSieve2 {
   static constexpr int NQ = LV::N - 1;
   QEntry (*q)[QCAP];
   int qn[NQ];
+  bool partial; /* a push ran with lanes switched off (must never happen) */
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
@@ -777,7 +785,10 @@ struct QSieve2 {
     const uint64_t bal = __ballot(pred);
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
     if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
-    qn[qi] += __popcll(bal);
+    /* the count is wave-uniform: keep it scalar, so that the queue loops
+       are uniform branches and every lane sees the same count */
+    qn[qi] = __builtin_amdgcn_readfirstlane(qn[qi] + __popcll(bal));
+    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -818,6 +829,8 @@ struct QSieve2 {
         run_level<L>(e, pred);
       }
       flush<L + 1>();
+    } else {
+      if (partial && lane == 0) atomicOr(a->err, 1u);
     }
   }
 };
@@ -828,26 +841,23 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
-  int64_t b = a.blk0 + tid;
-  bool has = b < a.blk1;
-  double xd = has ? (double)n53_of_block(a.kp, a.n43, b) + TWO52 : TWO52;
-  int i59 = 0;
-  while (__ballot(has) != 0) {
-    uint64_t m0 = 0, m1 = 0;
-    if (has) level0_pair<0, LV::B[1]>(xd, qs.ldsb, m0, m1);
-    qs.push(0, xd, m0, has && m0 != 0);
-    qs.template drain<0>();
-    qs.push(0, xd + TWO48, m1, has && m1 != 0);
-    qs.template drain<0>();
-    if (has) {
+  /* as in qsieve_blocks: uniform loops, the whole wave in every step */
+  for (int64_t base = a.blk0; base < a.blk1; base += stride) {
+    const int64_t b = base + tid;
+    const bool has = b < a.blk1;
+    if (__ballot(has) == 0) break;
+    double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
+    for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
+      uint64_t m0, m1;
+      level0_pair<0, LV::B[1]>(xd, qs.ldsb, m0, m1);
+      m0 = has ? m0 : 0;
+      m1 = has ? m1 : 0;
+      qs.push(0, xd, m0, m0 != 0);
+      qs.template drain<0>();
+      qs.push(0, xd + TWO48, m1, m1 != 0);
+      qs.template drain<0>();
       xd += S59d;
-      if (xd >= WRAP) xd -= MODd;
-      if (++i59 == AP26_N59_STEPS) {
-        i59 = 0;
-        b += stride;
-        has = b < a.blk1;
-        if (has) xd = (double)n53_of_block(a.kp, a.n43, b) + TWO52;
-      }
+      xd = xd >= WRAP ? xd - MODd : xd;
     }
   }
   qs.template flush<0>();
@@ -918,11 +928,13 @@ sieve_persistent2_kernel(PersistArgs p) {
     a.nsurv = p.nsurv;
     a.cap = p.cap;
     a.surv_job = p.surv_job;
+    a.err = p.err;
     a.job = jl;
     QSieve2<NL, LV> qs;
     qs.lane = threadIdx.x & 63;
     qs.q = L.queues[threadIdx.x >> 6];
     for (int i = 0; i < QSieve2<NL, LV>::NQ; i++) qs.qn[i] = 0;
+    qs.partial = false;
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;

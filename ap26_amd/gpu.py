@@ -78,6 +78,7 @@ SIGNATURES = {
     "ap26_sieve_span": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I64,
                                        ctypes.POINTER(_I64)]),
     "ap26_okok_table": (ctypes.c_int, [_P, _I32, _I32, _P]),
+    "ap26_batch_sieve_k": (ctypes.c_int, [_P, _I32, _I32, _I32, _P, _I64, ctypes.POINTER(_I64)]),
     "ap26_prp_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "ap26_prp_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_double)]),
@@ -199,6 +200,14 @@ class Context:
         rc = self._lib.ap26_sieve_span(self.h, K, shift, it0, it1, j0, j1, _ptr(out), cap,
                                        ctypes.byref(n))
         _check(rc, "ap26_sieve_span")
+        return out[:n.value].copy()
+
+    def batch_sieve_k(self, K, shift, iter_start=0, cap=1 << 26):
+        """Survivors of the batched engine's sieve for one K (parity against sieve_span)."""
+        out = np.zeros(cap, dtype=np.int64)
+        n = ctypes.c_int64()
+        rc = self._lib.ap26_batch_sieve_k(self.h, K, shift, iter_start, _ptr(out), cap, ctypes.byref(n))
+        _check(rc, "ap26_batch_sieve_k")
         return out[:n.value].copy()
 
     def okok_table(self, K, shift):

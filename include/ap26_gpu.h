@@ -126,6 +126,13 @@ int ap26_prp_batch_dev(ap26_ctx *ctx, const void *d_n, void *d_out, size_t count
 int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out,
                       int32_t cap, int32_t *n_out, int64_t *hash_totals, ap26_stats *stats);
 
+/* The batched engine's sieve alone for one K (shift iterations iter_start..9):
+   its survivors, in no particular order, for parity against ap26_sieve_span.
+   *n_out = total; AP26_E_OVERFLOW if > cap.  Tile of the same search as
+   ap26_search_batch (the reference's sieve, src/opencl/kernels/sieve.cl:12-156). */
+int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, int64_t *out,
+                       int64_t cap, int64_t *n_out);
+
 /* ---- serving: continuous batching of work units ------------------------
    Units are admitted while unit slots and device memory allow; every round
    takes the next K of each active unit in turn (round-robin), so units that

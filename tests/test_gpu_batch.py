@@ -61,6 +61,20 @@ def test_batch_many_rounds_against_golden(gpu_ctx, golden_index):
         assert got == recs
 
 
+@pytest.mark.parametrize("variant", ["10", "9", "12", "13"])
+def test_batch_sieve_matches_plain_survivors(variant, monkeypatch):
+    """The batched sieve (shift-pair and one-shift persistent kernels) keeps
+    exactly the plain path's survivors, including the last, partly filled
+    stride of n53 blocks of every work item and a resume at an odd ITER
+    (half of a shift pair skipped)."""
+    monkeypatch.setenv("AP26_BATCH_VARIANT", variant)
+    with gpu.Context(0) as ctx:
+        for K, sh, it in [(366384, 0, 0), (875859, 640, 3), (99999989, 1280, 9)]:
+            want = np.sort(ctx.sieve_span(K, sh, it, 10, 0, 10840, cap=1 << 26))
+            got = np.sort(ctx.batch_sieve_k(K, sh, it))
+            assert len(got) == len(want) and np.array_equal(got, want), (K, sh, it)
+
+
 def test_batch_edge_cases(gpu_ctx):
     sols, tot, st = gpu_ctx.search_batch([(29, 31, 0, 0)])  # 29, 31 skipped, 30 searched
     assert st.k_searched == 1 and st.k_skipped == 2
