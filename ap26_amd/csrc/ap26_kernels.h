@@ -370,9 +370,10 @@ struct QSieve {
     const uint64_t bal = __ballot(pred);
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
     if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
-    /* the count is wave-uniform: keep it scalar, so that the queue loops
-       are uniform branches and every lane sees the same count */
-    qn[qi] = __builtin_amdgcn_readfirstlane(qn[qi] + __popcll(bal));
+    /* every lane keeps the same count only if every lane runs every push:
+       the callers' loops are uniform, and a push on part of the wave is
+       reported (the queue would lose entries) */
+    qn[qi] += __popcll(bal);
     partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
     __builtin_amdgcn_wave_barrier();
   }
@@ -785,9 +786,10 @@ struct QSieve2 {
     const uint64_t bal = __ballot(pred);
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
     if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
-    /* the count is wave-uniform: keep it scalar, so that the queue loops
-       are uniform branches and every lane sees the same count */
-    qn[qi] = __builtin_amdgcn_readfirstlane(qn[qi] + __popcll(bal));
+    /* every lane keeps the same count only if every lane runs every push:
+       the callers' loops are uniform, and a push on part of the wave is
+       reported (the queue would lose entries) */
+    qn[qi] += __popcll(bal);
     partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
     __builtin_amdgcn_wave_barrier();
   }
