@@ -397,6 +397,9 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
       rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
       if (rc) return rc;
     }
+    hipLaunchKernelGGL(verify_kernel, dim3(64), dim3(256), 0, ctx->stream, b->d_sols,
+                       (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
     HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
@@ -420,6 +423,8 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     out[i].first = hs[i].first;
     out[i].range_id = hs[i].range_id;
     out[i].shift_iter = hs[i].shift_iter;
+    out[i].prp2_mask = hs[i].prp2;
+    out[i].valid_mask = hs[i].valid;
   }
   *n_out = (int32_t)nsol;
   if (hash_totals)
@@ -657,6 +662,9 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
   rc = launch_round(ctx, b, nullptr, nullptr);
   if (rc) return rc;
+  hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, b->d_sols,
+                     (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
                         ctx->stream));
@@ -674,7 +682,7 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   for (const DevSol &d : hs) {
     auto it = slot_owner.find(d.range_id);
     if (it == slot_owner.end()) return AP26_E_INTERNAL;
-    ap26_sol o{d.k, d.K, d.first, -1, d.shift_iter};
+    ap26_sol o{d.k, d.K, d.first, -1, d.shift_iter, d.prp2, d.valid};
     s->units[it->second].sols.push_back(o);
   }
   std::vector<int> cleared;

@@ -193,6 +193,44 @@ __device__ __forceinline__ bool sprp2(uint64_t n) {
   return false;
 }
 
+/* Strong probable-prime test to a small base a (2 <= a < 2^32), textbook
+ * Miller-Rabin as strong_prp(a, N) (src/common/PrimeQ_x86.h:114-153): n odd
+ * >= 3, n-1 = d*2^t; true iff a^d == 1 or a^(d*2^s) == -1 (mod n) for some
+ * 0 <= s < t; a == 0 (mod n) counts as passing, as in the host restatement.
+ * Montgomery form throughout (general multiply; verification is rare). */
+__device__ __noinline__ bool sprp_base(uint64_t n, uint32_t a) {
+  if (n < 3 || !(n & 1)) return false;
+  const uint64_t ar = (uint64_t)a % n;
+  if (ar == 0) return true;
+  const Mont M{n, neg_inverse(n)};
+  uint64_t d = n - 1;
+  const int t = __builtin_ctzll(d);
+  d >>= t;
+  const int bl = 64 - __builtin_clzll(n);
+  uint64_t x2 = (uint64_t)1 << (bl - 1); /* 2^65 mod n by doubling */
+  for (int i = 0; i < 66 - bl; i++) x2 = dbl_mod(x2, n);
+  const uint64_t one = (x2 & 1) ? (x2 >> 1) + (n >> 1) + 1 : (x2 >> 1);
+  const uint64_t mone = n - one;
+  /* a*R mod n by double-and-add over the bits of a */
+  uint64_t am = 0;
+  for (int i = 31 - __builtin_clz((uint32_t)ar); i >= 0; i--) {
+    am = dbl_mod(am, n);
+    if ((ar >> i) & 1) am = am >= n - one ? am - (n - one) : am + one;
+  }
+  uint64_t x = am;
+  for (int i = 62 - __builtin_clzll(d); i >= 0; i--) {
+    x = mont_mul(x, x, M);
+    if ((d >> i) & 1) x = mont_mul(x, am, M);
+  }
+  if (x == one || x == mone) return true;
+  for (int s = 1; s < t; s++) {
+    x = mont_mul(x, x, M);
+    if (x == mone) return true;
+    if (x == one) return false;
+  }
+  return false;
+}
+
 }  // namespace ap26
 
 #endif

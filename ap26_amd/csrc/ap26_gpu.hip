@@ -273,6 +273,8 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.ntests = ctx->d_cnt + C_NTESTS;
     hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
+    HIPCHK(hipGetLastError());
     st.walk_launches++;
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
     HIPCHK(hipMemcpyAsync(cnt, ctx->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
@@ -299,6 +301,8 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     out[i].first = hs[i].first;
     out[i].range_id = hs[i].range_id;
     out[i].shift_iter = hs[i].shift_iter;
+    out[i].prp2_mask = hs[i].prp2;
+    out[i].valid_mask = hs[i].valid;
   }
   *n_out = (int32_t)nsol;
   st.k_searched = 1;
@@ -349,6 +353,31 @@ int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_
   int64_t ncopy = (int64_t)nsurv < cap ? (int64_t)nsurv : cap;
   if (ncopy > 0) HIPCHK(hipMemcpy(out, ctx->d_surv, sizeof(int64_t) * (size_t)ncopy, hipMemcpyDeviceToHost));
   return (int64_t)nsurv > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
+int ap26_verify_sols(ap26_ctx *ctx, ap26_sol *sols, int32_t n) {
+  if (!ctx || n < 0 || (n > 0 && !sols)) return AP26_E_RANGE;
+  if (n > SOL_CAP) return AP26_E_RANGE;
+  if (n == 0) return AP26_OK;
+  for (int32_t i = 0; i < n; i++)
+    if (sols[i].k < 0 || sols[i].K < 1 || sols[i].K > 600000000 || sols[i].first < 0) return AP26_E_RANGE;
+  HIPCHK(hipSetDevice(ctx->device));
+  std::vector<DevSol> hs(n);
+  for (int32_t i = 0; i < n; i++) hs[i] = DevSol{sols[i].k, sols[i].K, sols[i].first, sols[i].range_id,
+                                                sols[i].shift_iter, 0u, 0u};
+  HIPCHK(hipMemcpyAsync(ctx->d_sols, hs.data(), sizeof(DevSol) * n, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned long long cnt = (unsigned long long)(uint32_t)n;
+  HIPCHK(hipMemcpyAsync(ctx->d_cnt + C_NSOL, &cnt, sizeof(cnt), hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols,
+                     (const unsigned int *)(ctx->d_cnt + C_NSOL), SOL_CAP);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(hs.data(), ctx->d_sols, sizeof(DevSol) * n, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  for (int32_t i = 0; i < n; i++) {
+    sols[i].prp2_mask = hs[i].prp2;
+    sols[i].valid_mask = hs[i].valid;
+  }
+  return AP26_OK;
 }
 
 int ap26_okok_table(ap26_ctx *ctx, int32_t K, int32_t SHIFT, uint64_t *out) {

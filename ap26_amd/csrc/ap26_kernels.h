@@ -27,6 +27,7 @@ struct DevSol {
   int32_t k, K;
   int64_t first;
   int32_t range_id, shift_iter;
+  uint32_t prp2, valid; /* bit j: term j is a base-2 SPRP / passes the 9 bases */
 };
 
 template <int B, int E, class F>
@@ -1093,6 +1094,37 @@ static __global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restr
 
 /* Clear counters with a kernel (not a memset node), so the sequence can be
    captured into a graph. */
+/* 9-base verification of reports (validate_ap26 / val_base2_ap26,
+   src/common/AP26.cpp:484-543): per report, bit j of prp2 says term
+   first + j*STEP is a base-2 strong probable prime, bit j of valid that it is
+   one to all of 2, 3, 5, 7, 11, 13, 17, 19, 23.  The host runs
+   ReportSolution's Non-Solution recursion on these bits (terms 0..31). */
+static __global__ void __launch_bounds__(256) verify_kernel(DevSol *__restrict__ sols,
+                                                            const unsigned int *__restrict__ nsol, int32_t cap) {
+  unsigned int n = *nsol;
+  if (n > (unsigned int)cap) n = (unsigned int)cap;
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const DevSol s = sols[i];
+    const int64_t step = (int64_t)s.K * AP26_PRIM23;
+    const int k = s.k < 32 ? s.k : 32;
+    uint32_t p2 = 0, v9 = 0;
+    for (int j = 0; j < k; j++) {
+      const int64_t N = s.first + (int64_t)j * step;
+      if (N < 3) continue;
+      if (!sprp_base((uint64_t)N, 2)) continue;
+      p2 |= 1u << j;
+      bool all = true;
+      static_for<0, 8>([&](auto IC) {
+        constexpr uint32_t B[8] = {3, 5, 7, 11, 13, 17, 19, 23};
+        if (all && !sprp_base((uint64_t)N, B[decltype(IC)::value])) all = false;
+      });
+      if (all) v9 |= 1u << j;
+    }
+    sols[i].prp2 = p2;
+    sols[i].valid = v9;
+  }
+}
+
 static __global__ void zero_kernel(unsigned long long *p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

@@ -26,11 +26,12 @@ NSHIFT = 10
 
 class Sol(ctypes.Structure):
     _fields_ = [("k", ctypes.c_int32), ("K", ctypes.c_int32), ("first", ctypes.c_int64),
-                ("range_id", ctypes.c_int32), ("shift_iter", ctypes.c_int32)]
+                ("range_id", ctypes.c_int32), ("shift_iter", ctypes.c_int32),
+                ("prp2_mask", ctypes.c_uint32), ("valid_mask", ctypes.c_uint32)]
 
 
 SOL_DTYPE = np.dtype([("k", "<i4"), ("K", "<i4"), ("first", "<i8"), ("range_id", "<i4"),
-                      ("shift_iter", "<i4")])
+                      ("shift_iter", "<i4"), ("prp2_mask", "<u4"), ("valid_mask", "<u4")])
 assert SOL_DTYPE.itemsize == ctypes.sizeof(Sol)
 
 
@@ -79,6 +80,7 @@ SIGNATURES = {
                                        ctypes.POINTER(_I64)]),
     "ap26_okok_table": (ctypes.c_int, [_P, _I32, _I32, _P]),
     "ap26_batch_sieve_k": (ctypes.c_int, [_P, _I32, _I32, _I32, _P, _I64, ctypes.POINTER(_I64)]),
+    "ap26_verify_sols": (ctypes.c_int, [_P, _P, _I32]),
     "ap26_prp_batch": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     "ap26_prp_batch_dev": (ctypes.c_int, [_P, _P, _P, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_double)]),
@@ -209,6 +211,12 @@ class Context:
         rc = self._lib.ap26_batch_sieve_k(self.h, K, shift, iter_start, _ptr(out), cap, ctypes.byref(n))
         _check(rc, "ap26_batch_sieve_k")
         return out[:n.value].copy()
+
+    def verify_sols(self, sols):
+        """Fill prp2_mask / valid_mask of a SOL_DTYPE array on the device."""
+        sols = np.ascontiguousarray(sols, dtype=SOL_DTYPE)
+        _check(self._lib.ap26_verify_sols(self.h, _ptr(sols), len(sols)), "ap26_verify_sols")
+        return sols
 
     def okok_table(self, K, shift):
         out = np.zeros(23693, dtype=np.uint64)

@@ -36,6 +36,7 @@ SIGNATURES = {
     "ap26h_write_state": (None, [_P, _I32, _I32]),
     "ap26h_checkpoint": (None, [_P, _I32, _I32, _I32]),
     "ap26h_report": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int64]),
+    "ap26h_report_masks": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32]),
     "ap26h_write_hash": (ctypes.c_int, [_P]),
     "ap26h_hash_word": (ctypes.c_uint64, [_I32, _I32, _I32]),
     "ap26h_cap_total": (_I32, [ctypes.c_int64]),
@@ -85,8 +86,12 @@ class Host:
     def checkpoint(self, K, it, force=0):
         self._lib.ap26h_checkpoint(ctypes.byref(self.st), K, force, it)
 
-    def report(self, k, K, first):
-        return self._lib.ap26h_report(ctypes.byref(self.st), k, K, first)
+    def report(self, k, K, first, prp2_mask=None, valid_mask=None):
+        """ReportSolution; with the device's per-term masks (ap26_sol) the
+        9-base verification is read from them instead of recomputed."""
+        if prp2_mask is None:
+            return self._lib.ap26h_report(ctypes.byref(self.st), k, K, first)
+        return self._lib.ap26h_report_masks(ctypes.byref(self.st), k, K, first, prp2_mask, valid_mask)
 
     def write_hash(self):
         return self._lib.ap26h_write_hash(ctypes.byref(self.st))

@@ -227,6 +227,77 @@ int ap26h_report(ap26_host *h, int32_t k, int32_t K, int64_t first) {
   return AP26H_OK;
 }
 
+/* validate_ap26 / val_base2_ap26 read from per-term bits computed on the
+   device: bit (off + j) of mask covers term j of this (sub-)AP.  Same order
+   as ap26h_validate: parity, then per term the range check and the test.
+   Terms past bit 31 are tested here. */
+static int32_t validate_bits(int32_t k, int32_t K, int64_t first, uint32_t mask, int off, int32_t nbases,
+                             int32_t *err) {
+  static const uint64_t bases[9] = {2, 3, 5, 7, 11, 13, 17, 19, 23};
+  if (err) *err = 0;
+  if (first % 2 == 0) return 0;
+  int64_t N = first;
+  int32_t j;
+  for (j = 0; j < k; j++) {
+    if (N < 0 || N >= (INT64_C(1) << 62)) {
+      fprintf(stderr, "ap26_host.c: modulus out of range.\n");
+      if (err) *err = AP26H_E_RANGE;
+      return j;
+    }
+    if (off + j < 32) {
+      if (!((mask >> (off + j)) & 1u)) return j;
+    } else {
+      for (int b = 0; b < nbases; b++)
+        if (!ap26h_sprp(bases[b], (uint64_t)N)) return j;
+    }
+    N += (int64_t)K * AP26_PRIM23;
+  }
+  return j;
+}
+
+static int report_bits(ap26_host *h, int32_t k, int32_t K, int64_t first, int off, uint32_t prp2_mask,
+                       uint32_t valid_mask) {
+  h->n_reports++;
+  h->result_hash += (int32_t)(first % 1000);
+  h->result_hash += k;
+  if (h->result_hash > AP26_MAXINTV) h->result_hash -= AP26_MAXINTV;
+
+  int32_t err = 0;
+  int32_t i = validate_bits(k, K, first, valid_mask, off, 9, &err);
+  if (err) return err;
+  if (i < k) {
+    h->n_nonsol++;
+    if (!h->quiet) printf("Non-Solution: %d %d %" PRId64 "\n", k, K, first);
+    int32_t i2 = validate_bits(k, K, first, prp2_mask, off, 1, &err);
+    if (err) return err;
+    if (i2 < k) {
+      printf("Error: Computation error, found invalid AP, exiting...\n");
+      fprintf(stderr, "Error: Computation error, found invalid AP\n");
+      return AP26H_E_INVALID;
+    }
+    int rc = report_bits(h, i, K, first, off, prp2_mask, valid_mask);
+    if (rc) return rc;
+    return report_bits(h, k - (i + 1), K, first + (int64_t)(i + 1) * K * AP26_PRIM23, off + i + 1, prp2_mask,
+                       valid_mask);
+  }
+  if (k >= h->min_report) {
+    int rc = open_results(h);
+    if (rc) return rc;
+    if (!h->quiet) printf("Solution: %d %d %" PRId64 "\n", k, K, first);
+    if (fprintf(h->results_file, "%d %d %" PRId64 "\n", k, K, first) < 0) {
+      fprintf(stderr, "Cannot write to %s !!!\n", h->results_name);
+      return AP26H_E_IO;
+    }
+    h->n_records++;
+  }
+  return AP26H_OK;
+}
+
+int ap26h_report_masks(ap26_host *h, int32_t k, int32_t K, int64_t first, uint32_t prp2_mask,
+                       uint32_t valid_mask) {
+  return report_bits(h, k, K, first, 0, prp2_mask, valid_mask);
+}
+
 /* ---- result hash (AP26.cpp:295-357) ------------------------------------- */
 
 uint64_t ap26h_hash_word(int32_t KMIN, int32_t KMAX, int32_t hash) {

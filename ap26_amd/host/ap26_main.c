@@ -35,7 +35,8 @@ static int report_all(ap26_host *h, ap26_sol *sols, int32_t n) {
      deterministic files */
   qsort(sols, (size_t)n, sizeof(ap26_sol), cmp_sol);
   for (int32_t i = 0; i < n; i++) {
-    int rc = ap26h_report(h, sols[i].k, sols[i].K, sols[i].first);
+    /* the device verified every term (9 bases): ReportSolution on its bits */
+    int rc = ap26h_report_masks(h, sols[i].k, sols[i].K, sols[i].first, sols[i].prp2_mask, sols[i].valid_mask);
     if (rc) return rc;
   }
   return 0;

@@ -120,7 +120,7 @@ def c4_sample(steps, per_unit=4):
 def report_all(hosts, sols):
     for s in sorted(sols.tolist(), key=lambda s: (s[3], s[2], s[0])):
         h = hosts[s[3]] if s[3] >= 0 else hosts[0]
-        if h.report(s[0], s[1], s[2]) != 0:
+        if h.report(s[0], s[1], s[2], s[5], s[6]) != 0:
             raise RuntimeError("report failed")
 
 
@@ -161,7 +161,7 @@ def run_plain(ctx, steps, warmup):
         tk = time.perf_counter()
         sols, st = ctx.search_k(K, SHIFT_C3)
         for s in sorted(sols.tolist(), key=lambda s: (s[2], s[0])):
-            if h.report(s[0], s[1], s[2]) != 0:
+            if h.report(s[0], s[1], s[2], s[5], s[6]) != 0:
                 raise RuntimeError("report failed")
         per_k.append(time.perf_counter() - tk)
         agg["sieve_ms"] += st.ms_sieve

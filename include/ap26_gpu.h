@@ -33,7 +33,11 @@ typedef struct {
   int32_t K;
   int64_t first;
   int32_t range_id;
-  int32_t shift_iter; /* which of the 10 shifts found it (0..9) */
+  int32_t shift_iter;  /* which of the 10 shifts found it (0..9) */
+  uint32_t prp2_mask;  /* bit j: first + j*K*23# is a base-2 strong PRP */
+  uint32_t valid_mask; /* bit j: ... and a strong PRP to 3, 5, ..., 23 too
+                          (validate_ap26, src/common/AP26.cpp:513-543);
+                          terms 0..31, computed on the device */
 } ap26_sol;
 
 /* One work unit: the reference's `AP26 KMIN KMAX SHIFT`, restricted to K in
@@ -132,6 +136,12 @@ int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *
    ap26_search_batch (the reference's sieve, src/opencl/kernels/sieve.cl:12-156). */
 int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, int64_t *out,
                        int64_t cap, int64_t *n_out);
+
+/* 9-base verification on the device of n caller-supplied reports (k, K,
+   first): fills prp2_mask and valid_mask of each (validate_ap26 and
+   val_base2_ap26, src/common/AP26.cpp:484-543).  The search entry points
+   fill them for their own reports. */
+int ap26_verify_sols(ap26_ctx *ctx, ap26_sol *sols, int32_t n);
 
 /* ---- serving: continuous batching of work units ------------------------
    Units are admitted while unit slots and device memory allow; every round

@@ -68,6 +68,13 @@ void ap26h_checkpoint(ap26_host *h, int32_t K, int32_t force, int32_t ITER);
 /* ReportSolution (AP26.cpp:549-602). */
 int ap26h_report(ap26_host *h, int32_t k, int32_t K, int64_t first);
 
+/* ReportSolution with the per-term verification done on the device: bit j of
+   prp2_mask / valid_mask says term first + j*K*23# is a strong probable prime
+   to base 2 / to all 9 bases (ap26_sol's masks).  Same files, hash, messages
+   and return codes as ap26h_report, without host SPRP work for terms 0..31. */
+int ap26h_report_masks(ap26_host *h, int32_t k, int32_t K, int64_t first, uint32_t prp2_mask,
+                       uint32_t valid_mask);
+
 /* write_hash (AP26.cpp:295-357): append the 16-hex-digit hash line. */
 int ap26h_write_hash(ap26_host *h);
 

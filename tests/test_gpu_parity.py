@@ -88,12 +88,50 @@ def test_full_k_against_reference_golden(gpu_ctx, golden_index, tmp_path):
                 continue
             sols, st = gpu_ctx.search_k(K, g["shift"])
             for s in sorted(sols.tolist(), key=lambda s: (s[2], s[0])):
-                assert h.report(s[0], s[1], s[2]) == 0
+                # ReportSolution on the device's 9-base verification bits
+                assert h.report(s[0], s[1], s[2], s[5], s[6]) == 0
         assert h.write_hash() == 0
         h.close()
         got = read_sol(str(d / "SOL-AP26.txt"))
         want = read_sol(os.path.join(GOLDEN, "ref_min10", g["name"] + ".sol"))
         assert got == want, g["name"]
+
+
+def _host_masks(k, K, first):
+    p2 = v9 = 0
+    for j in range(min(k, 32)):
+        n = first + j * K * PRIM23
+        if n >= 3 and H.sprp(2, n):
+            p2 |= 1 << j
+            if all(H.sprp(b, n) for b in (3, 5, 7, 11, 13, 17, 19, 23)):
+                v9 |= 1 << j
+    return p2, v9
+
+
+def test_device_verification_masks(gpu_ctx):
+    """9-base verification on the device (validate_ap26 / val_base2_ap26,
+    AP26.cpp:484-543) against the host's strong_prp restatement: every
+    report of two K, the reference's report vectors (APs holding base-2
+    strong pseudoprimes), and APs built around A001262 pseudoprimes."""
+    import json
+    sols, _ = gpu_ctx.search_k(44121555, 0)
+    rows = [(int(s["k"]), int(s["K"]), int(s["first"]), int(s["prp2_mask"]), int(s["valid_mask"])) for s in sols]
+    sols2, _ = gpu_ctx.search_k(875859, 640)
+    rows += [(int(s["k"]), int(s["K"]), int(s["first"]), int(s["prp2_mask"]), int(s["valid_mask"])) for s in sols2]
+    assert len(rows) > 300
+    for k, K, f, p2, v9 in rows:
+        assert (p2, v9) == _host_masks(k, K, f), (k, K, f)
+    vecs = json.load(open(os.path.join(GOLDEN, "report_vectors.json")))
+    extra = [tuple(r) for v in vecs for r in v["reports"]]
+    # pseudoprimes to base 2 that fail other bases, as first and middle terms
+    for psp in (2047, 3277, 4033, 4681, 8321, 3215031751, 3825123056546413051):
+        extra += [(12, 1, psp), (12, 3, psp - 5 * 3 * PRIM23) if psp > 5 * 3 * PRIM23 else (5, 1, psp)]
+    arr = np.zeros(len(extra), dtype=gpu.SOL_DTYPE)
+    for i, (k, K, f) in enumerate(extra):
+        arr[i]["k"], arr[i]["K"], arr[i]["first"] = k, K, f
+    out = gpu_ctx.verify_sols(arr)
+    for i, (k, K, f) in enumerate(extra):
+        assert (int(out[i]["prp2_mask"]), int(out[i]["valid_mask"])) == _host_masks(k, K, f), (k, K, f)
 
 
 def test_survivor_count_of_reference_comment(gpu_ctx):

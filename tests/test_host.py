@@ -32,6 +32,43 @@ def test_report_vectors_from_reference(tmp_path):
             assert h.st.n_nonsol == sum(len(x) for x in [v["nonsolutions"]])
 
 
+def host_masks(k, K, first):
+    """Per-term bits as the device computes them (base 2; all 9 bases), here
+    from the host's own strong_prp restatement."""
+    p2 = v9 = 0
+    for j in range(min(k, 32)):
+        n = first + j * K * 223092870
+        if n >= 3 and H.sprp(2, n):
+            p2 |= 1 << j
+            if all(H.sprp(b, n) for b in (3, 5, 7, 11, 13, 17, 19, 23)):
+                v9 |= 1 << j
+    return p2, v9
+
+
+def test_report_with_device_masks_equals_host_verification(tmp_path):
+    """ReportSolution on the device's per-term bits (ap26h_report_masks)
+    writes the same files, hash, Non-Solution splits and exit codes as the
+    host-side verification, on the reference's own vectors (pseudoprime
+    terms, recursion, the computation-error exit)."""
+    vecs = json.load(open(os.path.join(GOLDEN, "report_vectors.json")))
+    for i, v in enumerate(vecs):
+        res = []
+        for mode in ("host", "masks"):
+            d = tmp_path / f"{mode}{i}"
+            d.mkdir()
+            h = H.Host(1, 1, 0, str(d), min_report=10)
+            rcs = []
+            for k, K, first in v["reports"]:
+                rc = h.report(k, K, first) if mode == "host" else h.report(k, K, first, *host_masks(k, K, first))
+                rcs.append((rc, h.hash, h.st.n_nonsol, h.st.n_records))
+                if rc:
+                    break
+            h.close()
+            sol = d / "SOL-AP26.txt"
+            res.append((rcs, sol.read_text() if sol.exists() else ""))
+        assert res[0] == res[1], i
+
+
 def test_hash_lines_of_golden_units(golden_index):
     for g in golden_index:
         recs, hline = read_sol(os.path.join(GOLDEN, "ref_min10", g["name"] + ".sol"))
