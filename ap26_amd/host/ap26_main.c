@@ -11,12 +11,21 @@
  *                  reference CPU build does; default is one checkpoint per K
  *   -batch         search the whole unit with the batched persistent engine
  *   -q             no Solution/Checkpoint lines on stdout
+ *
+ * Batch manifest: `AP26 -manifest FILE [options]` reads work units, one
+ * "KMIN KMAX SHIFT" per line ('#' starts a comment), and runs them together on
+ * the batched engine.  Unit u keeps its files in ./unit_<KMIN>_<KMAX>_<SHIFT>/:
+ * the AP26-state.{a,b}.txt and SOL-AP26.txt that `AP26 KMIN KMAX SHIFT` run in
+ * that directory writes (same checkpoint sequence, records and hash line), so
+ * each unit resumes from its own checkpoint and compares with a single run.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <inttypes.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <errno.h>
+#include <sys/stat.h>
 #include <time.h>
 
 #include "ap26_gpu.h"
@@ -42,6 +51,171 @@ static int report_all(ap26_host *h, ap26_sol *sols, int32_t n) {
   return 0;
 }
 
+static int cmp_sol_unit(const void *a, const void *b) {
+  const ap26_sol *x = (const ap26_sol *)a, *y = (const ap26_sol *)b;
+  if (x->range_id != y->range_id) return x->range_id < y->range_id ? -1 : 1;
+  if (x->K != y->K) return x->K < y->K ? -1 : 1;
+  return cmp_sol(a, b);
+}
+
+/* Open a unit's state as AP26.cpp:789-832 does: resume (K, ITER) from the
+   newest state file, or start at KMIN. */
+static void unit_begin(ap26_host *h, int32_t *K, int32_t *ITER) {
+  if (ap26h_read_state(h, K, ITER)) {
+    printf("Resuming search from checkpoint.\n");
+    fprintf(stderr, "Resuming from checkpoint. K: %d ITER: %d\n", *K, *ITER);
+    /* any shift iteration 0..9 can be resumed (a superset of the reference
+       CPU builds' ITER rules, AP26.cpp:802-822) */
+    if (*ITER < 0 || *ITER > 9) {
+      printf("Checkpoint was from a different version. Restarting workunit.\n");
+      fprintf(stderr, "Checkpoint was from a different version. Restarting workunit.\n");
+      *K = h->KMIN;
+      *ITER = 0;
+      h->result_hash = 0;
+      h->last_trickle = (double)time(NULL);
+    }
+  } else {
+    printf("Beginning a new search with parameters from the command line\n");
+    *K = h->KMIN;
+    *ITER = 0;
+    h->result_hash = 0;
+    h->last_trickle = (double)time(NULL);
+    h->write_state_a_next = 1;
+  }
+}
+
+typedef struct {
+  ap26_host h;
+  int32_t K, ITER, Kend;
+  int done;
+} unit_t;
+
+#define MANIFEST_SOL_CAP (1 << 20)
+#define MANIFEST_K_PER_UNIT 16  /* searched K per unit per batched call */
+#define MANIFEST_K_PER_CALL 256 /* searched K per batched call */
+
+/* The batch manifest: every unit advances MANIFEST_K_PER_UNIT searched K per
+   batched call; the host then replays the single-unit loop for those K
+   (checkpoint before each K, its reports in order, K_DONE) from the call's
+   results, and closes a unit (final checkpoint, hash line) when its range
+   is done. */
+static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int quiet) {
+  FILE *f = fopen(path, "r");
+  if (!f) {
+    fprintf(stderr, "Cannot open manifest %s\n", path);
+    return EXIT_FAILURE;
+  }
+  unit_t *u = NULL;
+  int nu = 0, cap_u = 0, status = EXIT_SUCCESS;
+  char line[512];
+  while (fgets(line, sizeof(line), f)) {
+    char *c = line;
+    while (*c == ' ' || *c == '\t') c++;
+    if (*c == '#' || *c == '\n' || *c == '\r' || *c == 0) continue;
+    int32_t a, b, sh;
+    if (sscanf(c, "%d %d %d", &a, &b, &sh) != 3 || a < 1 || b < a || sh < 0) {
+      fprintf(stderr, "Bad manifest line: %s", line);
+      status = EXIT_FAILURE;
+      break;
+    }
+    if (nu == cap_u) {
+      cap_u = cap_u ? 2 * cap_u : 16;
+      unit_t *nuv = (unit_t *)realloc(u, sizeof(unit_t) * (size_t)cap_u);
+      if (!nuv) {
+        status = EXIT_FAILURE;
+        break;
+      }
+      u = nuv;
+    }
+    char dir[128];
+    snprintf(dir, sizeof(dir), "unit_%d_%d_%d", a, b, sh);
+    if (mkdir(dir, 0777) != 0 && errno != EEXIST) {
+      fprintf(stderr, "Cannot create %s\n", dir);
+      status = EXIT_FAILURE;
+      break;
+    }
+    unit_t *x = &u[nu++];
+    memset(x, 0, sizeof(*x));
+    ap26h_init(&x->h, a, b, sh, dir, min_report);
+    x->h.quiet = quiet;
+    unit_begin(&x->h, &x->K, &x->ITER);
+    ap26h_count(&x->h, x->K);
+  }
+  fclose(f);
+  ap26_sol *sols = status == EXIT_SUCCESS ? (ap26_sol *)malloc(sizeof(ap26_sol) * MANIFEST_SOL_CAP) : NULL;
+  ap26_range *r = status == EXIT_SUCCESS ? (ap26_range *)malloc(sizeof(ap26_range) * (size_t)(nu + 1)) : NULL;
+  int *owner = status == EXIT_SUCCESS ? (int *)malloc(sizeof(int) * (size_t)(nu + 1)) : NULL;
+  if (status == EXIT_SUCCESS && (!sols || !r || !owner)) status = EXIT_FAILURE;
+  printf("%d work units in %s\n", nu, path);
+  while (status == EXIT_SUCCESS) {
+    int nr = 0, kcall = 0;
+    for (int i = 0; i < nu && kcall < MANIFEST_K_PER_CALL; i++) {
+      unit_t *x = &u[i];
+      if (x->done) continue;
+      int cnt = 0;
+      int32_t k2 = x->K, last = x->K - 1;
+      for (; k2 <= x->h.KMAX && cnt < MANIFEST_K_PER_UNIT; k2++)
+        if (ap26h_will_search(k2)) {
+          cnt++;
+          last = k2;
+        }
+      x->Kend = cnt ? last : x->h.KMAX;
+      if (cnt) {
+        r[nr] = (ap26_range){x->K, x->Kend, x->h.SHIFT, x->ITER};
+        owner[nr++] = i;
+        kcall += cnt;
+      } else {
+        owner[nr] = i; /* nothing left to search: close below */
+        r[nr++] = (ap26_range){x->K, x->K - 1, x->h.SHIFT, 0};
+      }
+    }
+    if (nr == 0) break;
+    int32_t n = 0;
+    int rc = ap26_search_batch(ctx, r, nr, sols, MANIFEST_SOL_CAP, &n, NULL, NULL);
+    if (rc) {
+      fprintf(stderr, "Error: GPU search failed: %s\n", ap26_strerror(rc));
+      status = EXIT_FAILURE;
+      break;
+    }
+    qsort(sols, (size_t)n, sizeof(ap26_sol), cmp_sol_unit);
+    int32_t s = 0;
+    for (int ri = 0; ri < nr && status == EXIT_SUCCESS; ri++) {
+      unit_t *x = &u[owner[ri]];
+      for (int32_t K = x->K; K <= x->Kend && r[ri].kmax >= r[ri].kmin; K++) {
+        if (!ap26h_will_search(K)) continue;
+        ap26h_checkpoint(&x->h, K, 0, x->ITER);
+        for (; s < n && sols[s].range_id == ri && sols[s].K == K; s++) {
+          rc = ap26h_report_masks(&x->h, sols[s].k, sols[s].K, sols[s].first, sols[s].prp2_mask,
+                                  sols[s].valid_mask);
+          if (rc) break;
+        }
+        if (rc) {
+          fprintf(stderr, "Error: reporting for unit %d..%d failed\n", x->h.KMIN, x->h.KMAX);
+          status = EXIT_FAILURE;
+          break;
+        }
+        x->ITER = 0;
+        x->h.K_DONE++;
+      }
+      x->K = x->Kend + 1;
+      if (status == EXIT_SUCCESS && x->K > x->h.KMAX) {
+        ap26h_checkpoint(&x->h, x->K, 1, 0);
+        if (ap26h_write_hash(&x->h)) status = EXIT_FAILURE;
+        else fprintf(stderr, "Workunit %d %d %d complete.\n", x->h.KMIN, x->h.KMAX, x->h.SHIFT);
+        ap26h_close(&x->h);
+        x->done = 1;
+      }
+    }
+  }
+  for (int i = 0; i < nu; i++)
+    if (!u[i].done) ap26h_close(&u[i].h);
+  free(sols);
+  free(r);
+  free(owner);
+  free(u);
+  return status;
+}
+
 int main(int argc, char *argv[]) {
   int device = 0, min_report = AP26H_DEFAULT_MIN_REPORT, shiftckpt = 0, batch = 0, quiet = 0;
   printf("AP26 MI355X (HIP gfx950) 10-shift search, %s\n", ap26_version());
@@ -49,8 +223,28 @@ int main(int argc, char *argv[]) {
   for (int i = 0; i < argc; i++) fprintf(stderr, "%s ", argv[i]);
   fprintf(stderr, "\n");
 
+  const char *manifest = NULL;
+  for (int i = 1; i + 1 < argc; i++)
+    if (!strcmp(argv[i], "-manifest")) manifest = argv[i + 1];
+  if (manifest) {
+    for (int i = 1; i < argc; i++) {
+      if (!strcmp(argv[i], "-device") && i + 1 < argc) device = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-report") && i + 1 < argc) min_report = atoi(argv[++i]);
+      else if (!strcmp(argv[i], "-q")) quiet = 1;
+    }
+    ap26_ctx *mctx = NULL;
+    int mrc = ap26_open(device, NULL, &mctx);
+    if (mrc) {
+      fprintf(stderr, "Error: cannot open GPU %d: %s\n", device, ap26_strerror(mrc));
+      return EXIT_FAILURE;
+    }
+    int st = run_manifest(mctx, manifest, min_report, quiet);
+    ap26_close(mctx);
+    return st;
+  }
   if (argc < 4) {
-    printf("Usage: %s KMIN KMAX SHIFT [-device N] [-report N] [-shiftckpt] [-batch] [-q]\n", argv[0]);
+    printf("Usage: %s KMIN KMAX SHIFT [-device N] [-report N] [-shiftckpt] [-batch] [-q]\n"
+           "       %s -manifest FILE [-device N] [-report N] [-q]\n", argv[0], argv[0]);
     return EXIT_FAILURE;
   }
   int32_t KMIN = 0, KMAX = 0, SHIFT = 0;
@@ -80,27 +274,7 @@ int main(int argc, char *argv[]) {
   ap26h_init(&h, KMIN, KMAX, SHIFT, NULL, min_report);
   h.quiet = quiet;
   int32_t K, ITER;
-  if (ap26h_read_state(&h, &K, &ITER)) {
-    printf("Resuming search from checkpoint.\n");
-    fprintf(stderr, "Resuming from checkpoint. K: %d ITER: %d\n", K, ITER);
-    /* any shift iteration 0..9 can be resumed (a superset of the reference
-       CPU builds' ITER rules, AP26.cpp:802-822) */
-    if (ITER < 0 || ITER > 9) {
-      printf("Checkpoint was from a different version. Restarting workunit.\n");
-      fprintf(stderr, "Checkpoint was from a different version. Restarting workunit.\n");
-      K = KMIN;
-      ITER = 0;
-      h.result_hash = 0;
-      h.last_trickle = (double)time(NULL);
-    }
-  } else {
-    printf("Beginning a new search with parameters from the command line\n");
-    K = KMIN;
-    ITER = 0;
-    h.result_hash = 0;
-    h.last_trickle = (double)time(NULL);
-    h.write_state_a_next = 1;
-  }
+  unit_begin(&h, &K, &ITER);
 
   ap26_ctx *ctx = NULL;
   int rc = ap26_open(device, NULL, &ctx);

@@ -105,3 +105,49 @@ def test_batch_flag_same_output(golden_index, tmp_path):
     assert r.returncode == 0, r.stderr
     want = _units_hash(golden_index, ["c1_875859", "c1_875860"], 875858, 875860)
     assert read_sol(str(tmp_path / "SOL-AP26.txt")) == want
+
+
+def _state_fields(d):
+    """State files without the trickle timestamp (the reference's last field)."""
+    out = {}
+    for f in ("AP26-state.a.txt", "AP26-state.b.txt"):
+        p = d / f
+        out[f] = p.read_text().split()[:6] if p.exists() else None
+    return out
+
+
+def test_manifest_units_match_single_runs(golden_index, tmp_path):
+    """`AP26 -manifest FILE` runs several units on the batched engine and
+    leaves in unit_<KMIN>_<KMAX>_<SHIFT>/ the same SOL file (records and hash
+    line, byte for byte) and state files as `AP26 KMIN KMAX SHIFT` run alone in
+    that directory; one unit resumes from a checkpoint left by an earlier run."""
+    units = [(366384, 366384, 0), (875858, 875860, 640), (976061, 976062, 0), (29, 31, 0)]
+    single = tmp_path / "single"
+    single.mkdir()
+    for a, b, sh in units:
+        d = single / f"unit_{a}_{b}_{sh}"
+        d.mkdir()
+        if (a, b) == (875858, 875860):  # resumed after K=875859, as the reference writes it
+            T59 = int(golden(golden_index, "c1_875859")["hash"][8:], 16)
+            (d / "AP26-state.a.txt").write_text(f"875858 875860 640 875860 0 {T59} 1234.000000\n")
+        r = run([a, b, sh, "-report", 10], d)
+        assert r.returncode == 0, r.stderr
+    batch = tmp_path / "batch"
+    batch.mkdir()
+    d0 = batch / "unit_875858_875860_640"
+    d0.mkdir()
+    T59 = int(golden(golden_index, "c1_875859")["hash"][8:], 16)
+    (d0 / "AP26-state.a.txt").write_text(f"875858 875860 640 875860 0 {T59} 1234.000000\n")
+    (batch / "units.txt").write_text("# KMIN KMAX SHIFT\n" + "".join(f"{a} {b} {sh}\n" for a, b, sh in units))
+    r = run(["-manifest", "units.txt", "-report", 10], batch)
+    assert r.returncode == 0, r.stderr
+    for a, b, sh in units:
+        name = f"unit_{a}_{b}_{sh}"
+        s_sol = (single / name / "SOL-AP26.txt").read_text()
+        b_sol = (batch / name / "SOL-AP26.txt").read_text()
+        assert sorted(b_sol.splitlines()[:-1]) == sorted(s_sol.splitlines()[:-1]), name
+        assert b_sol.splitlines()[-1] == s_sol.splitlines()[-1], name
+        assert _state_fields(batch / name) == _state_fields(single / name), name
+    # and against the reference's own runs
+    want = read_sol(os.path.join(GOLDEN, "ref_min10", "t366384.sol"))
+    assert read_sol(str(batch / "unit_366384_366384_0" / "SOL-AP26.txt")) == want
