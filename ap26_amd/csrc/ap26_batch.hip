@@ -734,6 +734,20 @@ extern "C" int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out,
   return AP26_OK;
 }
 
+extern "C" int ap26_server_progress(ap26_server *s, int64_t ticket, double *fraction, int64_t *k_done,
+                                    int64_t *k_total) {
+  if (!s) return AP26_E_RANGE;
+  auto it = s->units.find(ticket);
+  if (it == s->units.end()) return AP26_E_RANGE;
+  const SUnit &u = it->second;
+  const int64_t total = (int64_t)u.ks.size();
+  const int64_t done = u.done ? total : (int64_t)u.next - u.outstanding;
+  if (k_done) *k_done = done;
+  if (k_total) *k_total = total;
+  if (fraction) *fraction = total > 0 ? (double)done / (double)total : 1.0;
+  return AP26_OK;
+}
+
 extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int64_t *rounds,
                                  int32_t *k_per_round, int32_t *active) {
   if (!s) return AP26_E_RANGE;

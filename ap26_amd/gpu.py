@@ -93,6 +93,8 @@ SIGNATURES = {
     "ap26_server_result": (ctypes.c_int, [_P, _I64, _P, _I32, ctypes.POINTER(_I32),
                                           ctypes.POINTER(_I64), ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(_I32)]),
+    "ap26_server_progress": (ctypes.c_int, [_P, _I64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64),
+                                            ctypes.POINTER(_I64)]),
     "ap26_server_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
                                          ctypes.POINTER(_I64), ctypes.POINTER(_I32),
                                          ctypes.POINTER(_I32)]),
@@ -328,6 +330,13 @@ class Server:
         if not done.value:
             return None
         return out[:n.value].copy(), h.value, lat.value
+
+    def progress(self, ticket):
+        """(fraction done, searched K done, searched K in the unit)."""
+        f, kd, kt = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.ap26_server_progress(self.h, ticket, ctypes.byref(f), ctypes.byref(kd), ctypes.byref(kt)),
+               "ap26_server_progress")
+        return f.value, kd.value, kt.value
 
     def stats(self):
         ud, kd, rd = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()

@@ -22,7 +22,8 @@ class HostState(ctypes.Structure):
                 ("quiet", ctypes.c_int32), ("n_reports", ctypes.c_int64),
                 ("n_records", ctypes.c_int64), ("n_nonsol", ctypes.c_int64),
                 ("results_file", ctypes.c_void_p), ("state_a", ctypes.c_char * 1024),
-                ("state_b", ctypes.c_char * 1024), ("results_name", ctypes.c_char * 1024)]
+                ("state_b", ctypes.c_char * 1024), ("results_name", ctypes.c_char * 1024),
+                ("metrics_file", ctypes.c_void_p), ("t0_wall", ctypes.c_double), ("t0_cpu", ctypes.c_double)]
 
 
 _P = ctypes.POINTER(HostState)
@@ -38,6 +39,8 @@ SIGNATURES = {
     "ap26h_report": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int64]),
     "ap26h_report_masks": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_int64, ctypes.c_uint32, ctypes.c_uint32]),
     "ap26h_write_hash": (ctypes.c_int, [_P]),
+    "ap26h_progress": (ctypes.c_double, [_P, _I32]),
+    "ap26h_open_metrics": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ap26h_hash_word": (ctypes.c_uint64, [_I32, _I32, _I32]),
     "ap26h_cap_total": (_I32, [ctypes.c_int64]),
     "ap26h_sprp": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64]),
@@ -92,6 +95,12 @@ class Host:
         if prp2_mask is None:
             return self._lib.ap26h_report(ctypes.byref(self.st), k, K, first)
         return self._lib.ap26h_report_masks(ctypes.byref(self.st), k, K, first, prp2_mask, valid_mask)
+
+    def progress(self, it=0):
+        return self._lib.ap26h_progress(ctypes.byref(self.st), it)
+
+    def open_metrics(self, path):
+        return self._lib.ap26h_open_metrics(ctypes.byref(self.st), str(path).encode())
 
     def write_hash(self):
         return self._lib.ap26h_write_hash(ctypes.byref(self.st))

@@ -39,6 +39,39 @@ void ap26h_close(ap26_host *h) {
     fclose(h->results_file);
     h->results_file = NULL;
   }
+  if (h->metrics_file) {
+    fclose(h->metrics_file);
+    h->metrics_file = NULL;
+  }
+}
+
+static double wall_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+static double cpu_seconds(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_PROCESS_CPUTIME_ID, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+double ap26h_progress(const ap26_host *h, int32_t ITER) {
+  if (h->K_COUNT <= 0) return 1.0;
+  return (double)(h->K_DONE * 10 + ITER) / (double)(h->K_COUNT * 10);
+}
+
+int ap26h_open_metrics(ap26_host *h, const char *path) {
+  if (h->metrics_file) fclose(h->metrics_file);
+  h->metrics_file = fopen(path, "a");
+  if (!h->metrics_file) {
+    fprintf(stderr, "Cannot open %s !!!\n", path);
+    return AP26H_E_IO;
+  }
+  h->t0_wall = wall_seconds();
+  h->t0_cpu = cpu_seconds();
+  return AP26H_OK;
 }
 
 int ap26h_will_search(int32_t K) {
@@ -127,6 +160,17 @@ void ap26h_checkpoint(ap26_host *h, int32_t K, int32_t force, int32_t ITER) {
   if (!h->quiet)
     printf("Checkpoint: KMIN:%d KMAX:%d SHIFT:%d K:%d ITER:%d\n", h->KMIN, h->KMAX, h->SHIFT, K,
            ITER);
+  if (h->metrics_file) {
+    const double prog = ap26h_progress(h, ITER);
+    if (!h->quiet) printf("progress: %.2f%%\n", prog * 100.0);
+    fprintf(h->metrics_file,
+            "{\"KMIN\": %d, \"KMAX\": %d, \"SHIFT\": %d, \"K\": %d, \"ITER\": %d, \"progress\": %f, "
+            "\"cputime\": %f, \"runtime\": %f, \"k_done\": %d, \"k_count\": %d, \"result_hash\": %d, "
+            "\"reports\": %" PRId64 ", \"records\": %" PRId64 ", \"non_solutions\": %" PRId64 "}\n",
+            h->KMIN, h->KMAX, h->SHIFT, K, ITER, prog, cpu_seconds() - h->t0_cpu, wall_seconds() - h->t0_wall,
+            h->K_DONE, h->K_COUNT, h->result_hash, h->n_reports, h->n_records, h->n_nonsol);
+    fflush(h->metrics_file);
+  }
 }
 
 /* ---- verification (AP26.cpp:484-543, PrimeQ_x86.h:114-153) -------------- */

@@ -11,6 +11,10 @@
  *                  reference CPU build does; default is one checkpoint per K
  *   -batch         search the whole unit with the batched persistent engine
  *   -q             no Solution/Checkpoint lines on stdout
+ *   -metrics FILE  append a progress record (progress, cputime, runtime, K,
+ *                  ITER, counts: the trickle-up fields as metrics) to FILE at
+ *                  every checkpoint; with -manifest, FILE inside each unit's
+ *                  directory
  *
  * Batch manifest: `AP26 -manifest FILE [options]` reads work units, one
  * "KMIN KMAX SHIFT" per line ('#' starts a comment), and runs them together on
@@ -99,7 +103,7 @@ typedef struct {
    (checkpoint before each K, its reports in order, K_DONE) from the call's
    results, and closes a unit (final checkpoint, hash line) when its range
    is done. */
-static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int quiet) {
+static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int quiet, const char *metrics) {
   FILE *f = fopen(path, "r");
   if (!f) {
     fprintf(stderr, "Cannot open manifest %s\n", path);
@@ -140,6 +144,14 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
     x->h.quiet = quiet;
     unit_begin(&x->h, &x->K, &x->ITER);
     ap26h_count(&x->h, x->K);
+    if (metrics) {
+      char mp[1024];
+      snprintf(mp, sizeof(mp), "%s/%s", dir, metrics);
+      if (ap26h_open_metrics(&x->h, mp)) {
+        status = EXIT_FAILURE;
+        break;
+      }
+    }
   }
   fclose(f);
   ap26_sol *sols = status == EXIT_SUCCESS ? (ap26_sol *)malloc(sizeof(ap26_sol) * MANIFEST_SOL_CAP) : NULL;
@@ -223,9 +235,11 @@ int main(int argc, char *argv[]) {
   for (int i = 0; i < argc; i++) fprintf(stderr, "%s ", argv[i]);
   fprintf(stderr, "\n");
 
-  const char *manifest = NULL;
-  for (int i = 1; i + 1 < argc; i++)
+  const char *manifest = NULL, *metrics = NULL;
+  for (int i = 1; i + 1 < argc; i++) {
     if (!strcmp(argv[i], "-manifest")) manifest = argv[i + 1];
+    if (!strcmp(argv[i], "-metrics")) metrics = argv[i + 1];
+  }
   if (manifest) {
     for (int i = 1; i < argc; i++) {
       if (!strcmp(argv[i], "-device") && i + 1 < argc) device = atoi(argv[++i]);
@@ -238,13 +252,13 @@ int main(int argc, char *argv[]) {
       fprintf(stderr, "Error: cannot open GPU %d: %s\n", device, ap26_strerror(mrc));
       return EXIT_FAILURE;
     }
-    int st = run_manifest(mctx, manifest, min_report, quiet);
+    int st = run_manifest(mctx, manifest, min_report, quiet, metrics);
     ap26_close(mctx);
     return st;
   }
   if (argc < 4) {
-    printf("Usage: %s KMIN KMAX SHIFT [-device N] [-report N] [-shiftckpt] [-batch] [-q]\n"
-           "       %s -manifest FILE [-device N] [-report N] [-q]\n", argv[0], argv[0]);
+    printf("Usage: %s KMIN KMAX SHIFT [-device N] [-report N] [-shiftckpt] [-batch] [-q] [-metrics FILE]\n"
+           "       %s -manifest FILE [-device N] [-report N] [-q] [-metrics FILE]\n", argv[0], argv[0]);
     return EXIT_FAILURE;
   }
   int32_t KMIN = 0, KMAX = 0, SHIFT = 0;
@@ -264,6 +278,8 @@ int main(int argc, char *argv[]) {
       batch = 1;
     } else if (!strcmp(argv[i], "-q")) {
       quiet = 1;
+    } else if (!strcmp(argv[i], "-metrics") && i + 1 < argc) {
+      i++; /* taken above */
     } else if (!strcmp(argv[i], "-sse2") || !strcmp(argv[i], "-sse41") || !strcmp(argv[i], "-avx") ||
                !strcmp(argv[i], "-avx2")) {
       fprintf(stderr, "ignoring CPU option %s (GPU build)\n", argv[i]);
@@ -275,6 +291,7 @@ int main(int argc, char *argv[]) {
   h.quiet = quiet;
   int32_t K, ITER;
   unit_begin(&h, &K, &ITER);
+  if (metrics && ap26h_open_metrics(&h, metrics)) return EXIT_FAILURE;
 
   ap26_ctx *ctx = NULL;
   int rc = ap26_open(device, NULL, &ctx);

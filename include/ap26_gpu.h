@@ -166,6 +166,10 @@ int ap26_server_pump(ap26_server *s, int32_t *completed);
 int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out, int32_t cap, int32_t *n_out,
                        int64_t *hash_total, double *latency_ms, int32_t *done);
 /* Counters: units admitted/finished, K searched, rounds, K per round. */
+/* Progress of one admitted unit: searched K finished / searched K in the
+   unit (the reference's fraction done, AP26.cpp:636-641, at K granularity). */
+int ap26_server_progress(ap26_server *s, int64_t ticket, double *fraction, int64_t *k_done, int64_t *k_total);
+
 int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int64_t *rounds,
                       int32_t *k_per_round, int32_t *active);
 

@@ -39,6 +39,8 @@ typedef struct {
   int64_t n_nonsol;         /* Non-Solution splits */
   FILE *results_file;
   char state_a[1024], state_b[1024], results_name[1024];
+  FILE *metrics_file;       /* progress records, one JSON line per checkpoint */
+  double t0_wall, t0_cpu;   /* run start (seconds) for runtime / cputime */
 } ap26_host;
 
 /* Initialise for a work unit; dir (may be NULL or "") prefixes the file names
@@ -74,6 +76,16 @@ int ap26h_report(ap26_host *h, int32_t k, int32_t K, int64_t first);
    and return codes as ap26h_report, without host SPRP work for terms 0..31. */
 int ap26h_report_masks(ap26_host *h, int32_t k, int32_t K, int64_t first, uint32_t prp2_mask,
                        uint32_t valid_mask);
+
+/* Fraction of the unit done at (K_DONE, ITER): (K_DONE*10 + ITER) /
+   (K_COUNT*10), 1 for an empty unit (AP26.cpp:636-641, boinc_fraction_done). */
+double ap26h_progress(const ap26_host *h, int32_t ITER);
+
+/* Progress records, the trickle-up message's fields (AP26.cpp:251-287:
+   progress, cputime, runtime) as metrics: from this call on, every
+   checkpoint appends one JSON line to path (append mode) and prints the
+   non-BOINC "progress: x.xx%" line (AP26.cpp:240-248) unless quiet. */
+int ap26h_open_metrics(ap26_host *h, const char *path);
 
 /* write_hash (AP26.cpp:295-357): append the 16-hex-digit hash line. */
 int ap26h_write_hash(ap26_host *h);

@@ -144,6 +144,22 @@ def test_server_heavy_k(gpu_ctx):
         assert res[i][1] == sum(f % 1000 + k for k, _, f in want)
 
 
+def test_server_progress(gpu_ctx):
+    """Per-unit progress while a unit runs: searched K done / in the unit."""
+    with gpu.Server(gpu_ctx, max_units=4, mem_budget_bytes=1 << 30) as srv:  # one K per round
+        t = srv.submit(875858, 875862, 640)
+        assert srv.progress(t) == (0.0, 0, 4)
+        seen, r = [], None
+        while r is None:
+            srv.pump()
+            r = srv.result(t)
+            if r is None:
+                seen.append(srv.progress(t))
+        assert [s[1] for s in seen] == [1, 2, 3] and seen[0][0] == 0.25
+        with pytest.raises(gpu.Ap26Error):
+            srv.progress(t)  # collected
+
+
 def test_server_admission_limit(gpu_ctx):
     with gpu.Server(gpu_ctx, max_units=2, mem_budget_bytes=1 << 30) as srv:
         a = srv.submit(366384, 366384, 0)

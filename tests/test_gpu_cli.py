@@ -151,3 +151,26 @@ def test_manifest_units_match_single_runs(golden_index, tmp_path):
     # and against the reference's own runs
     want = read_sol(os.path.join(GOLDEN, "ref_min10", "t366384.sol"))
     assert read_sol(str(batch / "unit_366384_366384_0" / "SOL-AP26.txt")) == want
+
+
+def test_metrics_records(tmp_path):
+    """-metrics: one progress record per checkpoint (the trickle-up fields),
+    progress non-decreasing to 1.0; the non-BOINC progress line on stdout."""
+    import json
+    r = run([875858, 875862, 640, "-report", 10, "-metrics", "m.jsonl"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    recs = [json.loads(l) for l in open(tmp_path / "m.jsonl")]
+    ks = [K for K in range(875858, 875863) if H.will_search(K)]
+    assert [x["K"] for x in recs] == ks + [875863]
+    prog = [x["progress"] for x in recs]
+    assert prog == sorted(prog) and prog[0] == 0.0 and prog[-1] == 1.0
+    assert recs[-1]["result_hash"] == int((tmp_path / "SOL-AP26.txt").read_text().splitlines()[-1][8:], 16)
+    assert "progress: 100.00%" in r.stdout
+    m = tmp_path / "m2"
+    m.mkdir()
+    (m / "u.txt").write_text("875858 875862 640\n366384 366384 0\n")
+    r = run(["-manifest", "u.txt", "-report", 10, "-metrics", "m.jsonl", "-q"], m)
+    assert r.returncode == 0, r.stderr
+    for d in ("unit_875858_875862_640", "unit_366384_366384_0"):
+        recs = [json.loads(l) for l in open(m / d / "m.jsonl")]
+        assert recs[-1]["progress"] == 1.0 and recs[-1]["k_done"] == recs[-1]["k_count"]

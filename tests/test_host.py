@@ -157,3 +157,26 @@ def test_count():
 def test_validate_range_error():
     j, err = H.validate(2, 1, (1 << 62) + 1, 1)
     assert err == H.AP26H_E_RANGE
+
+
+def test_progress_and_metrics_records(tmp_path):
+    """Fraction done (AP26.cpp:636-641) and the trickle-up fields written as
+    JSON lines at each checkpoint (AP26.cpp:251-287)."""
+    h = H.Host(875858, 875873, 640, str(tmp_path), min_report=10)
+    h.count(875858)
+    assert h.st.K_COUNT == 13 and h.progress(0) == 0.0
+    assert h.open_metrics(tmp_path / "m.jsonl") == 0
+    h.checkpoint(875859, 0)
+    h.st.K_DONE = 5
+    h.checkpoint(875866, 3)
+    h.st.K_DONE = 13
+    h.checkpoint(875874, 0, force=1)
+    h.close()
+    recs = [json.loads(l) for l in open(tmp_path / "m.jsonl")]
+    assert [r["K"] for r in recs] == [875859, 875866, 875874]
+    assert [round(r["progress"], 6) for r in recs] == [0.0, round(53 / 130, 6), 1.0]
+    assert all(r["runtime"] >= 0 and r["cputime"] >= 0 for r in recs)
+    assert recs[-1]["k_done"] == recs[-1]["k_count"] == 13
+    e = H.Host(29, 29, 0, str(tmp_path / "x"), min_report=10)
+    e.count(29)
+    assert e.progress(0) == 1.0  # nothing searched: done
