@@ -42,7 +42,10 @@ PersistVariant make_pvariant2() {
   return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV>};
 }
 
-/* 1..3, 9: one shift per item; 0 (default), 10..14: shift pairs */
+/* 1..3, 9: one shift per item; 0 (default = 28), 10..34: shift pairs.  Level 0
+   (every position, both shifts of a pair per 16-byte read) runs on the first
+   18..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
+   so level 0 pays off further than on one shift (61..131). */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -61,7 +64,20 @@ PersistVariant pick_pvariant(int id) {
     case 19: return make_pvariant2<15, 16, Levels4>();
     case 20: return make_pvariant2<18, 16, Levels<0, 15, 22, 29, 45, 83>>();
     case 21: return make_pvariant2<15, 16, Levels<0, 15, 22, 29, 45, 83>>();
-    default: return make_pvariant2<29, 16, Levels4>();
+    case 22: return make_pvariant2<29, 16, Levels<0, 18, 29, 45, 83>>();
+    case 23: return make_pvariant2<29, 16, Levels<0, 20, 29, 45, 83>>();
+    case 24: return make_pvariant2<29, 16, Levels<0, 22, 29, 45, 83>>();
+    case 25: return make_pvariant2<29, 16, Levels<0, 20, 29, 41, 83>>();
+    case 26: return make_pvariant2<29, 16, Levels<0, 20, 29, 51, 83>>();
+    case 27: return make_pvariant2<29, 16, Levels<0, 24, 29, 45, 83>>();
+    case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
+    case 29: return make_pvariant2<29, 16, Levels<0, 19, 29, 45, 83>>();
+    case 30: return make_pvariant2<29, 16, Levels<0, 20, 29, 55, 83>>();
+    case 31: return make_pvariant2<29, 16, Levels<0, 18, 27, 45, 83>>();
+    case 32: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>>();
+    case 33: return make_pvariant2<27, 16, Levels<0, 20, 29, 51, 83>>();
+    case 34: return make_pvariant2<29, 16, Levels<0, 19, 29, 51, 83>>();
+    default: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
   }
 }
 
@@ -766,7 +782,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {10, 11, 12, 13, 14, 9, 1, 2};
+static const int TUNE_VARIANTS[] = {28, 32, 22, 26, 10, 12, 14, 9, 1};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

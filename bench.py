@@ -264,9 +264,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
-    ap.add_argument("--full-tune", action="store_true", help="autotune over all 24 candidates")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 27 candidates (9 variants x 3 work-item sizes)")
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (10, 85)")
+                    help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (0, 85)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -275,7 +275,7 @@ def main():
 
     ctx = gpu.Context(0)
     if args.no_tune:
-        tune_best = {"variant": int(os.environ.get("AP26_BATCH_VARIANT", "10")),
+        tune_best = {"variant": int(os.environ.get("AP26_BATCH_VARIANT", "0")),
                      "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "85"))}
         tune_all = []
     else:

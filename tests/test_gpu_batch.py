@@ -61,7 +61,7 @@ def test_batch_many_rounds_against_golden(gpu_ctx, golden_index):
         assert got == recs
 
 
-@pytest.mark.parametrize("variant", ["10", "9", "12", "13"])
+@pytest.mark.parametrize("variant", ["0", "10", "9", "12", "13", "32"])
 def test_batch_sieve_matches_plain_survivors(variant, monkeypatch):
     """The batched sieve (shift-pair and one-shift persistent kernels) keeps
     exactly the plain path's survivors, including the last, partly filled
