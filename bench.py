@@ -204,14 +204,14 @@ def run_prp(ctx, count=1 << 26, seed=26):
                          "frac": tops / PEAK_INT32_TOPS}}
 
 
-def run_serving(ctx, kps, n_units=12, budget=3 << 30, seed=27):
+def run_serving(ctx, kps, n_units=12, budget=3 << 30, seed=27, load=0.5):
     rng = np.random.default_rng(seed)
     lens = np.floor(np.exp(rng.uniform(0, np.log(16), n_units))).astype(int)
     starts = rng.integers(1_000_000, 100_000_001, n_units)
     shifts = rng.choice([0, 640, 1280], n_units)
     units = [(int(s), int(s + L - 1), int(sh)) for s, L, sh in zip(starts, lens, shifts)]
     mean_k = np.mean([sum(will_search(K) for K in range(a, b + 1)) for a, b, _ in units])
-    rate = 0.5 * kps / max(mean_k, 1e-9)   # units per second
+    rate = load * kps / max(mean_k, 1e-9)   # units per second
     gaps = rng.exponential(1.0 / rate, n_units)
     arrive = np.cumsum(gaps) - gaps[0]
     lat = []
@@ -236,7 +236,7 @@ def run_serving(ctx, kps, n_units=12, budget=3 << 30, seed=27):
         wall = time.perf_counter() - t0
         st = srv.stats()
     lat = np.array(lat)
-    return {"units": n_units, "rate_units_per_s": rate, "load": 0.5, "k_per_s": st["k_done"] / wall,
+    return {"units": n_units, "rate_units_per_s": rate, "load": load, "k_per_s": st["k_done"] / wall,
             "k_per_round": st["k_per_round"], "p50_ms": float(np.percentile(lat, 50)),
             "p95_ms": float(np.percentile(lat, 95)), "p99_ms": float(np.percentile(lat, 99)),
             "unit_len": "floor(exp(U(0, ln 16)))"}
@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
     ap.add_argument("--full-tune", action="store_true", help="autotune over all 27 candidates (9 variants x 3 work-item sizes)")
+    ap.add_argument("--serve-loads", default="0.5",
+                    help="offered loads of the serving leg, as fractions of the batched K/s (e.g. 0.25,0.5,1.0)")
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (0, 85)")
     args = ap.parse_args()
@@ -331,7 +333,10 @@ def main():
     if not args.no_prp:
         line["prp"] = run_prp(ctx)
     if not args.no_serve:
-        line["serving"] = run_serving(ctx, kps)
+        loads = [float(x) for x in args.serve_loads.split(",")]
+        line["serving"] = run_serving(ctx, kps, load=loads[0])
+        if len(loads) > 1:
+            line["serving_sweep"] = [run_serving(ctx, kps, load=x) for x in loads]
     ctx.close()
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
