@@ -275,6 +275,7 @@ walk:
   w.hash_tot = b->d_hash;
   w.overflow = b->d_cnt + 4;
   w.nlive = b->d_cnt + 5;
+  w.err = (unsigned int *)(b->d_cnt + 6);
   hipLaunchKernelGGL(batch_walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
   hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_rs, b->d_cnt + 0,
                      (unsigned int *)(b->d_cnt + 3));

@@ -271,6 +271,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.nsol = (unsigned int *)(ctx->d_cnt + C_NSOL);
     w.sol_cap = SOL_CAP;
     w.ntests = ctx->d_cnt + C_NTESTS;
+    w.err = (unsigned int *)(ctx->d_cnt + C_ERR);
     hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);

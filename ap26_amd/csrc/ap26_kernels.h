@@ -946,6 +946,95 @@ sieve_persistent2_kernel(PersistArgs p) {
   }
 }
 
+/* ---- compacted walk ------------------------------------------------------ */
+/* The walk of a survivor c counts base-2 SPRPs forward from c+5*STEP; only a
+ * run of >= 10 extends backward from c+4*STEP while the term is positive, and
+ * is reported as (k, m+STEP) (src/cpu/cpusse2.cpp:570-590,
+ * src/opencl/kernels/checkn.cl:160-202).  About 72 % of runs stop at the first
+ * term, so one thread per survivor leaves most lanes idle while a few runs go
+ * on.  Here runs wait in a per-wave LDS queue and every pass tests one term
+ * on 64 dense lanes: runs that go on are queued again, finished ones are
+ * closed (backward extension and report, rare, inline). */
+struct WEntry {
+  int64_t c;   /* survivor */
+  int64_t m;   /* next forward term to test */
+  int32_t k;   /* SPRPs found forward so far */
+  int32_t job; /* batched engine: job index (0 on the plain path) */
+};
+
+constexpr int WALK_QCAP = 128;
+
+template <class StepOf, class Close>
+struct WalkQueue {
+  WEntry *q; /* this wave's WALK_QCAP entries */
+  int qn = 0;
+  int lane;
+  bool partial = false;
+  unsigned long long tests = 0;
+  StepOf step_of;
+  Close close;
+
+  __device__ __forceinline__ void push(const WEntry &e, bool pred) {
+    const uint64_t bal = __ballot(pred);
+    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    if (pred) q[qn + pre] = e;
+    qn += __popcll(bal);
+    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  /* test one term of each lane's run (pred lanes), requeue runs that go on */
+  __device__ __forceinline__ void step(WEntry e, bool pred) {
+    bool go = false;
+    if (pred) {
+      tests++;
+      go = sprp2((uint64_t)e.m);
+      if (go) {
+        e.k++;
+        e.m += step_of(e.job);
+      }
+    }
+    push(e, pred && go);
+    if (pred && !go && e.k >= 10) close(e, tests);
+  }
+
+  __device__ __forceinline__ void drain() {
+    while (qn >= 64) {
+      qn -= 64;
+      const WEntry e = q[qn + lane];
+      __builtin_amdgcn_wave_barrier();
+      step(e, true);
+    }
+  }
+
+  __device__ __forceinline__ void flush() {
+    while (qn > 0) {
+      const int cnt = qn < 64 ? qn : 64;
+      qn -= cnt;
+      const bool pred = lane < cnt;
+      WEntry e{0, 0, 0, 0};
+      if (pred) e = q[qn + lane];
+      __builtin_amdgcn_wave_barrier();
+      step(e, pred);
+    }
+  }
+};
+
+/* backward extension of a forward run of k >= 10 ending before e.m; returns
+   k and the first term */
+__device__ __forceinline__ int walk_back(const WEntry &e, int64_t STEP, int64_t *first, unsigned long long &tests) {
+  int k = e.k;
+  int64_t m = e.c + 4 * STEP;
+  while (m > 0) {
+    tests++;
+    if (!sprp2((uint64_t)m)) break;
+    k++;
+    m -= STEP;
+  }
+  *first = m + STEP;
+  return k;
+}
+
 struct BatchWalkArgs {
   const BatchJobDev *__restrict__ jobs;
   const RoundState *__restrict__ rs;
@@ -960,46 +1049,23 @@ struct BatchWalkArgs {
   unsigned long long *__restrict__ hash_tot; /* per range: exact sum of first%1000 + k */
   unsigned long long *__restrict__ overflow; /* largest survivor count beyond cap */
   unsigned long long *__restrict__ nlive;    /* survivors walked, over all rounds */
+  unsigned int *__restrict__ err;            /* a queue step ran on part of a wave */
 };
 
-__device__ __forceinline__ int walk_one(int64_t c, int64_t STEP, int64_t *first, unsigned long long &tests) {
-  int k = 0;
-  int64_t m = c + 5 * STEP;
-  while (true) {
-    tests++;
-    if (!sprp2((uint64_t)m)) break;
-    k++;
-    m += STEP;
-  }
-  if (k >= 10) {
-    m = c + 4 * STEP;
-    while (m > 0) {
-      tests++;
-      if (!sprp2((uint64_t)m)) break;
-      k++;
-      m -= STEP;
-    }
-  }
-  *first = m + STEP;
-  return k;
-}
-
 static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a) {
+  __shared__ WEntry s_q[4][WALK_QCAP];
   const RoundState rs = *a.rs;
   unsigned long long n = *a.nsurv;
   if (n > (unsigned long long)a.cap) {
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicMax(a.overflow, n);
     n = (unsigned long long)a.cap;
   }
-  unsigned long long tests = 0, live = 0;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    const int64_t c = a.surv[i];
-    if (c < 0) continue; /* slot reserved but unused */
-    live++;
-    const BatchJobDev &job = a.jobs[rs.base + a.surv_job[i]];
+  const BatchJobDev *jobs = a.jobs + rs.base;
+  auto step_of = [&](int32_t job) { return jobs[job].kp.STEP; };
+  auto close = [&](const WEntry &e, unsigned long long &tests) {
+    const BatchJobDev &job = jobs[e.job];
     int64_t first;
-    const int k = walk_one(c, job.kp.STEP, &first, tests);
+    const int k = walk_back(e, job.kp.STEP, &first, tests);
     if (k >= 10) {
       atomicAdd(&a.hash_tot[job.range_id], (unsigned long long)(first % 1000 + k));
       const unsigned int idx = atomicAdd(a.nsol, 1u);
@@ -1009,12 +1075,35 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
         s.K = job.kp.K;
         s.first = first;
         s.range_id = job.range_id;
-        s.shift_iter = (int32_t)((c / AP26_MOD - job.kp.SHIFT) / 64);
+        s.shift_iter = (int32_t)((e.c / AP26_MOD - job.kp.SHIFT) / 64);
+        s.prp2 = s.valid = 0;
         a.sols[idx] = s;
       }
     }
+  };
+  WalkQueue<decltype(step_of), decltype(close)> wq{s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0,
+                                                   step_of, close};
+  unsigned long long live = 0;
+  const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
+  /* uniform loops: the queue needs the whole wave in every step */
+  for (unsigned long long base = wave * 64; base < n; base += nwaves * 64) {
+    const unsigned long long i = base + (unsigned long long)wq.lane;
+    const int64_t c = i < n ? a.surv[i] : -1;
+    const bool has = c >= 0; /* -1: slot reserved but unused */
+    WEntry e{0, 0, 0, 0};
+    if (has) {
+      e.job = a.surv_job[i];
+      e.c = c;
+      e.m = c + 5 * jobs[e.job].kp.STEP;
+      live++;
+    }
+    wq.step(e, has);
+    wq.drain();
   }
-  if (tests) atomicAdd(a.ntests, tests);
+  wq.flush();
+  if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
+  if (wq.tests) atomicAdd(a.ntests, wq.tests);
   if (live) atomicAdd(a.nlive, live);
 }
 
@@ -1042,46 +1131,50 @@ struct WalkArgs {
   unsigned int *__restrict__ nsol;
   int32_t sol_cap;
   unsigned long long *__restrict__ ntests;
+  unsigned int *__restrict__ err; /* a queue step ran on part of a wave */
 };
 
 static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
+  __shared__ WEntry s_q[4][WALK_QCAP];
   unsigned long long n = *a.nsurv;
   if (n > (unsigned long long)a.cap) n = (unsigned long long)a.cap;
-  unsigned long long tests = 0;
-  for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (unsigned long long)gridDim.x * blockDim.x) {
-    const int64_t c = a.surv[i];
-    int k = 0;
-    int64_t m = c + 5 * a.STEP;
-    while (true) {
-      tests++;
-      if (!sprp2((uint64_t)m)) break;
-      k++;
-      m += a.STEP;
-    }
+  const int64_t STEP = a.STEP;
+  auto step_of = [&](int32_t) { return STEP; };
+  auto close = [&](const WEntry &e, unsigned long long &tests) {
+    int64_t first;
+    const int k = walk_back(e, STEP, &first, tests);
     if (k >= 10) {
-      m = c + 4 * a.STEP;
-      while (m > 0) {
-        tests++;
-        if (!sprp2((uint64_t)m)) break;
-        k++;
-        m -= a.STEP;
-      }
-      if (k >= 10) {
-        unsigned int idx = atomicAdd(a.nsol, 1u);
-        if (idx < (unsigned int)a.sol_cap) {
-          DevSol s;
-          s.k = k;
-          s.K = a.K;
-          s.first = m + a.STEP;
-          s.range_id = a.range_id;
-          s.shift_iter = (int32_t)((c / AP26_MOD - a.SHIFT) / 64);
-          a.sols[idx] = s;
-        }
+      const unsigned int idx = atomicAdd(a.nsol, 1u);
+      if (idx < (unsigned int)a.sol_cap) {
+        DevSol s;
+        s.k = k;
+        s.K = a.K;
+        s.first = first;
+        s.range_id = a.range_id;
+        s.shift_iter = (int32_t)((e.c / AP26_MOD - a.SHIFT) / 64);
+        s.prp2 = s.valid = 0;
+        a.sols[idx] = s;
       }
     }
+  };
+  WalkQueue<decltype(step_of), decltype(close)> wq{s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0,
+                                                   step_of, close};
+  const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
+  for (unsigned long long base = wave * 64; base < n; base += nwaves * 64) {
+    const unsigned long long i = base + (unsigned long long)wq.lane;
+    const bool has = i < n;
+    WEntry e{0, 0, 0, 0};
+    if (has) {
+      e.c = a.surv[i];
+      e.m = e.c + 5 * STEP;
+    }
+    wq.step(e, has);
+    wq.drain();
   }
-  if (tests) atomicAdd(a.ntests, tests);
+  wq.flush();
+  if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
+  if (wq.tests) atomicAdd(a.ntests, wq.tests);
 }
 
 /* ---- batched SPRP ------------------------------------------------------ */
