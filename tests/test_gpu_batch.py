@@ -192,3 +192,37 @@ def test_autotune_picks_and_caches(tmp_path, monkeypatch):
     with gpu.Context(0) as ctx:  # adopts the cached choice
         sols, tot, st = ctx.search_batch([(875859, 875859, 640, 0)])
         assert key(sols) == want
+
+
+def test_batch_equals_plain_on_mixed_workload_sample(gpu_ctx):
+    """At the benchmark's scale: the first K of the 256-unit mixed workload
+    (SHIFT 0/640/1280, K from 1e6 to 1e8; 32 searched K, two rounds) give the
+    same reports and per-unit hash totals on the batched engine as on the
+    plain path, and the batched survivor count equals the plain one."""
+    rng = np.random.default_rng(26)
+    n = 256
+    lens = np.clip(np.floor(np.exp(rng.uniform(0, np.log(4096), n))).astype(int), 1, 4096)
+    starts = rng.integers(1_000_000, 100_000_001, n)
+    shifts = rng.choice([0, 640, 1280], n)
+    units, got = [], 0
+    for s, L, sh in zip(starts, lens, shifts):
+        ks = [K for K in range(int(s), int(s + L)) if will_search(K)][:4]
+        if not ks:
+            continue
+        ks = ks[:32 - got]
+        units.append((ks[0], ks[-1], int(sh), 0))
+        got += len([K for K in range(ks[0], ks[-1] + 1) if will_search(K)])
+        if got >= 32:
+            break
+    sols, tot, st = gpu_ctx.search_batch(units)
+    plain_surv = 0
+    for i, (a, b, sh, _) in enumerate(units):
+        want = []
+        for K in range(a, b + 1):
+            if will_search(K):
+                s, pst = gpu_ctx.search_k(K, sh)
+                plain_surv += pst.survivors
+                want += [(int(x["k"]), int(x["K"]), int(x["first"])) for x in s]
+        assert key([x for x in sols if x["range_id"] == i]) == sorted(want), units[i]
+        assert tot[i] == sum(f % 1000 + k for k, _, f in want), units[i]
+    assert st.survivors == plain_surv and st.k_searched == got
