@@ -27,7 +27,7 @@ using namespace ap26;
 constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* initial survivor buffer: 67M (~40M per K) */
 constexpr int SOL_CAP = 1 << 16;
 constexpr int NCOUNT = 8; /* device counters */
-enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_ERR = 7 };
+enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_LIVE = 3, C_ERR = 7 };
 
 using SieveFn = void (*)(SieveArgs);
 
@@ -44,6 +44,11 @@ SieveVariant make_variant() {
 template <int NL, int WAVES, class LV, bool VC, unsigned G = 0>
 SieveVariant make_qvariant() {
   return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC, G>};
+}
+
+template <int NL, int WAVES, class LV>
+SieveVariant make_q2variant() {
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV>};
 }
 
 

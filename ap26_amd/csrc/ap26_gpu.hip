@@ -18,10 +18,10 @@ using namespace ap26;
 
 #include "ap26_ctx.h"
 
-/* Sieve variants: 0 = default queued sieve, 1 = simple S=1, 2 = simple S=2,
-   10+ = queued-sieve tuning candidates (LDS primes, waves per CU, levels, VGPR
-   constants). */
-#define AP26_NVARIANTS 31
+/* Sieve variants: 0 = default (31: shift-pair queued sieve, level 0 on 18
+   primes), 1 = simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve
+   tuning candidates (LDS primes, waves per CU, levels, VGPR constants),
+   31..33 = shift-pair queued sieves. */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -47,7 +47,10 @@ SieveVariant pick_variant(int id) {
     case 28: return make_qvariant<38, 16, Levels4, false, 0x7fff>();
     case 29: return make_qvariant<38, 16, Levels4, false, 0x0fff>();
     case 30: return make_qvariant<38, 16, Levels4, false, 0x00ff>();
-    default: return make_qvariant<38, 16, Levels4, false>();
+    case 31: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>>();
+    case 32: return make_q2variant<29, 16, Levels<0, 20, 31, 51, 83>>();
+    case 33: return make_q2variant<29, 16, Levels4>();
+    default: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>>();
   }
 }
 
@@ -272,6 +275,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.sol_cap = SOL_CAP;
     w.ntests = ctx->d_cnt + C_NTESTS;
     w.err = (unsigned int *)(ctx->d_cnt + C_ERR);
+    w.nlive = ctx->d_cnt + C_LIVE;
     hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
@@ -307,7 +311,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
   }
   *n_out = (int32_t)nsol;
   st.k_searched = 1;
-  st.survivors = (int64_t)nsurv;
+  st.survivors = (int64_t)cnt[C_LIVE];
   st.prp_tests = (int64_t)cnt[C_NTESTS];
   st.reports = nsol;
   st.ms_tables = elapsed_ms(ctx->ev[0], ctx->ev[1]);
@@ -350,10 +354,18 @@ int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_
     rc = ensure_surv(ctx, (int64_t)(nsurv + (nsurv >> 3)));
     if (rc) return rc;
   }
-  *n_out = (int64_t)nsurv;
-  int64_t ncopy = (int64_t)nsurv < cap ? (int64_t)nsurv : cap;
-  if (ncopy > 0) HIPCHK(hipMemcpy(out, ctx->d_surv, sizeof(int64_t) * (size_t)ncopy, hipMemcpyDeviceToHost));
-  return (int64_t)nsurv > cap ? AP26_E_OVERFLOW : AP26_OK;
+  /* the shift-pair sieve leaves unused slots (-1) at the end of each wave's
+     last chunk: keep the survivors only */
+  std::vector<int64_t> all((size_t)nsurv);
+  if (nsurv) HIPCHK(hipMemcpy(all.data(), ctx->d_surv, sizeof(int64_t) * (size_t)nsurv, hipMemcpyDeviceToHost));
+  int64_t n = 0;
+  for (int64_t v : all)
+    if (v >= 0) {
+      if (n < cap) out[n] = v;
+      n++;
+    }
+  *n_out = n;
+  return n > cap ? AP26_E_OVERFLOW : AP26_OK;
 }
 
 int ap26_verify_sols(ap26_ctx *ctx, ap26_sol *sols, int32_t n) {

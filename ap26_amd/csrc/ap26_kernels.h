@@ -750,7 +750,7 @@ __device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const Sie
           pre < cur.left ? cur.base + (unsigned long long)pre : fresh + (unsigned long long)(pre - cur.left);
       if (idx < (unsigned long long)a.cap) {
         a.surv[idx] = nbase + (int64_t)bit * AP26_MOD;
-        a.surv_job[idx] = a.job;
+        if (a.surv_job) a.surv_job[idx] = a.job;
       }
     }
     if (cnt > cur.left) {
@@ -946,6 +946,35 @@ sieve_persistent2_kernel(PersistArgs p) {
   }
 }
 
+/* One launch per (K, shift pair): the plain path with the shift-pair sieve
+   (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
+   both shifts), survivors in per-wave slot chunks as in the batched engine. */
+template <int NL, int WAVES, class LV>
+static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
+sieve_queue2_kernel(SieveArgs a) {
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
+  constexpr int LMAIN2 = 2 * PT.off[NL];
+  __shared__ QSieve2Lds<NL, WAVES, LV> L;
+  __shared__ SurvCursor s_cur[WAVES];
+  for (int i = threadIdx.x; i < LMAIN2; i += blockDim.x) L.tab[i] = a.tab[i];
+  for (int i = threadIdx.x; i < 180; i += blockDim.x) L.tab[LMAIN2 + i] = a.tab[2 * NOK + i];
+  SurvCursor &sc = s_cur[threadIdx.x >> 6];
+  sc = SurvCursor{0, 0};
+  __syncthreads();
+  QSieve2<NL, LV> qs;
+  qs.lane = threadIdx.x & 63;
+  qs.q = L.queues[threadIdx.x >> 6];
+  for (int i = 0; i < QSieve2<NL, LV>::NQ; i++) qs.qn[i] = 0;
+  qs.partial = false;
+  qs.ldsb = (const char *)L.tab;
+  qs.gtb = (const char *)a.tab;
+  qs.a = &a;
+  qs.cur = &sc;
+  qsieve2_blocks<NL, LV>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x);
+  close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
+}
+
 /* ---- compacted walk ------------------------------------------------------ */
 /* The walk of a survivor c counts base-2 SPRPs forward from c+5*STEP; only a
  * run of >= 10 extends backward from c+4*STEP while the term is positive, and
@@ -1132,6 +1161,7 @@ struct WalkArgs {
   int32_t sol_cap;
   unsigned long long *__restrict__ ntests;
   unsigned int *__restrict__ err; /* a queue step ran on part of a wave */
+  unsigned long long *__restrict__ nlive; /* survivors walked */
 };
 
 static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
@@ -1161,13 +1191,16 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
                                                    step_of, close};
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
+  unsigned long long live = 0;
   for (unsigned long long base = wave * 64; base < n; base += nwaves * 64) {
     const unsigned long long i = base + (unsigned long long)wq.lane;
-    const bool has = i < n;
+    const int64_t c = i < n ? a.surv[i] : -1;
+    const bool has = c >= 0; /* -1: slot reserved but unused */
     WEntry e{0, 0, 0, 0};
     if (has) {
-      e.c = a.surv[i];
-      e.m = e.c + 5 * STEP;
+      e.c = c;
+      e.m = c + 5 * STEP;
+      live++;
     }
     wq.step(e, has);
     wq.drain();
@@ -1175,6 +1208,7 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   wq.flush();
   if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
   if (wq.tests) atomicAdd(a.ntests, wq.tests);
+  if (live) atomicAdd(a.nlive, live);
 }
 
 /* ---- batched SPRP ------------------------------------------------------ */

@@ -52,8 +52,9 @@ typedef struct {
 
 /* Tuning knobs (0 = library default / autotuner cache). */
 typedef struct {
-  int32_t sieve_variant;    /* 0 queued-compaction sieve (default), 1 simple
-                               per-shift sieve, 2 simple sieve, 2 shifts per pass */
+  int32_t sieve_variant;    /* 0 default (shift-pair queued-compaction sieve),
+                               1 simple per-shift sieve, 2 simple sieve with 2
+                               shifts per pass, 10..33 tuning candidates */
   int32_t block;            /* (reported) threads per workgroup of the sieve */
   int32_t blocks_per_cu;    /* resident workgroups per CU (grid = CUs x this) */
   int32_t verbose;

@@ -52,7 +52,7 @@ SPANS = [(366384, 0, 0, 10, 0, 3), (875859, 640, 0, 10, 100, 102), (48620635, 64
          (61 * 67 * 71 * 73, 0, 0, 10, 3000, 3002)]
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 23, 32])
 @pytest.mark.parametrize("span", SPANS)
 def test_sieve_span_parity(variant, span):
     with gpu.Context(0, sieve_variant=variant) as ctx:
