@@ -414,8 +414,10 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
       rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(verify_kernel, dim3(64), dim3(256), 0, ctx->stream, b->d_sols,
-                       (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+    hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream,
+                       b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+    hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream,
+                       b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
     HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
@@ -679,8 +681,10 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
   rc = launch_round(ctx, b, nullptr, nullptr);
   if (rc) return rc;
-  hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, b->d_sols,
-                     (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+  hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream,
+                     b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+  hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream,
+                     b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,

@@ -278,7 +278,8 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.nlive = ctx->d_cnt + C_LIVE;
     hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
+    hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
+    hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
     HIPCHK(hipGetLastError());
     st.walk_launches++;
     HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
@@ -292,7 +293,6 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     rc = ensure_surv(ctx, (int64_t)(cnt[C_NSURV] + (cnt[C_NSURV] >> 3)));
     if (rc) return rc;
   }
-  const unsigned long long nsurv = cnt[C_NSURV];
   const unsigned int nsol = (unsigned int)(cnt[C_NSOL] & 0xffffffffu);
   if (nsol > (unsigned int)SOL_CAP) return AP26_E_INTERNAL;
   std::vector<DevSol> hs(nsol);
@@ -381,8 +381,10 @@ int ap26_verify_sols(ap26_ctx *ctx, ap26_sol *sols, int32_t n) {
   HIPCHK(hipMemcpyAsync(ctx->d_sols, hs.data(), sizeof(DevSol) * n, hipMemcpyHostToDevice, ctx->stream));
   const unsigned long long cnt = (unsigned long long)(uint32_t)n;
   HIPCHK(hipMemcpyAsync(ctx->d_cnt + C_NSOL, &cnt, sizeof(cnt), hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(verify_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols,
-                     (const unsigned int *)(ctx->d_cnt + C_NSOL), SOL_CAP);
+  hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream,
+                     ctx->d_sols, (const unsigned int *)(ctx->d_cnt + C_NSOL), SOL_CAP);
+  hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream,
+                     ctx->d_sols, (const unsigned int *)(ctx->d_cnt + C_NSOL), SOL_CAP);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(hs.data(), ctx->d_sols, sizeof(DevSol) * n, hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));

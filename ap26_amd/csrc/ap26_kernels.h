@@ -1225,31 +1225,41 @@ static __global__ void __launch_bounds__(256) prp_kernel(const uint64_t *__restr
    src/common/AP26.cpp:484-543): per report, bit j of prp2 says term
    first + j*STEP is a base-2 strong probable prime, bit j of valid that it is
    one to all of 2, 3, 5, 7, 11, 13, 17, 19, 23.  The host runs
-   ReportSolution's Non-Solution recursion on these bits (terms 0..31). */
+   ReportSolution's Non-Solution recursion on these bits (terms 0..31).
+   One thread per (report, term, base): verify_kernel ORs the base-2 passes
+   into prp2 and the other bases' failures into valid (both start at 0), and
+   verify_finish_kernel turns valid into prp2 & ~failures. */
 static __global__ void __launch_bounds__(256) verify_kernel(DevSol *__restrict__ sols,
                                                             const unsigned int *__restrict__ nsol, int32_t cap) {
   unsigned int n = *nsol;
   if (n > (unsigned int)cap) n = (unsigned int)cap;
-  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const DevSol s = sols[i];
-    const int64_t step = (int64_t)s.K * AP26_PRIM23;
-    const int k = s.k < 32 ? s.k : 32;
-    uint32_t p2 = 0, v9 = 0;
-    for (int j = 0; j < k; j++) {
-      const int64_t N = s.first + (int64_t)j * step;
-      if (N < 3) continue;
-      if (!sprp_base((uint64_t)N, 2)) continue;
-      p2 |= 1u << j;
-      bool all = true;
-      static_for<0, 8>([&](auto IC) {
-        constexpr uint32_t B[8] = {3, 5, 7, 11, 13, 17, 19, 23};
-        if (all && !sprp_base((uint64_t)N, B[decltype(IC)::value])) all = false;
-      });
-      if (all) v9 |= 1u << j;
+  const uint64_t total = (uint64_t)n * 32 * 9;
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned int i = (unsigned int)(t / (32 * 9));
+    const int r = (int)(t - (uint64_t)i * (32 * 9));
+    const int j = r / 9, bi = r - 9 * (r / 9);
+    const DevSol &s = sols[i];
+    if (j >= s.k || j >= 32) continue;
+    const int64_t N = s.first + (int64_t)j * ((int64_t)s.K * AP26_PRIM23);
+    if (N < 3) continue;
+    constexpr uint32_t B[9] = {2, 3, 5, 7, 11, 13, 17, 19, 23};
+    const bool pass = sprp_base((uint64_t)N, B[bi]);
+    if (bi == 0) {
+      if (pass) atomicOr(&sols[i].prp2, 1u << j);
+    } else if (!pass) {
+      atomicOr(&sols[i].valid, 1u << j);
     }
-    sols[i].prp2 = p2;
-    sols[i].valid = v9;
   }
+}
+
+static __global__ void __launch_bounds__(256) verify_finish_kernel(DevSol *__restrict__ sols,
+                                                                   const unsigned int *__restrict__ nsol,
+                                                                   int32_t cap) {
+  unsigned int n = *nsol;
+  if (n > (unsigned int)cap) n = (unsigned int)cap;
+  for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    sols[i].valid = sols[i].prp2 & ~sols[i].valid;
 }
 
 static __global__ void zero_kernel(unsigned long long *p, int n) {
