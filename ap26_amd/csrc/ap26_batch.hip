@@ -37,9 +37,9 @@ PersistVariant make_pvariant() {
   return PersistVariant{NL, WAVES, 1, &sieve_persistent_kernel<NL, WAVES, LV>};
 }
 
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, int V = 0>
 PersistVariant make_pvariant2() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV>};
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, V>};
 }
 
 /* 1..3, 9: one shift per item; 0 (default = 28), 10..34: shift pairs.  Level 0
@@ -77,6 +77,10 @@ PersistVariant pick_pvariant(int id) {
     case 32: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>>();
     case 33: return make_pvariant2<27, 16, Levels<0, 20, 29, 51, 83>>();
     case 34: return make_pvariant2<29, 16, Levels<0, 19, 29, 51, 83>>();
+    case 35: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 1>();
+    case 36: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 2>();
+    case 37: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 3>();
+    case 38: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>, 3>();
     default: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
   }
 }
@@ -324,6 +328,7 @@ static BatchJobDev make_job(int32_t K, int32_t shift, int32_t it0, int32_t range
   j.kp.S59 = hk.S[6];
   j.kp.K = K;
   j.kp.SHIFT = shift;
+  for (int v = 0; v < NVW; v++) host_vwin(K, VW_P[v], j.kp.vw[v]);
   j.it0 = it0;
   j.range_id = range_id;
   return j;

@@ -46,9 +46,9 @@ SieveVariant make_qvariant() {
   return SieveVariant{1, NL, 64 * WAVES, &sieve_queue_kernel<NL, WAVES, LV, VC, G>};
 }
 
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, int V = 0>
 SieveVariant make_q2variant() {
-  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV>};
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, V>};
 }
 
 

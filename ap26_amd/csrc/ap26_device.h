@@ -68,11 +68,13 @@ __device__ __forceinline__ uint32_t lo32(double d) {
 /* Byte offset SCALE*r of the centred entry of n59 relative to the entry of
  * residue 0 (SCALE = bytes per centred entry), given x = n59 as a double and
  * xs = SCALE * (low 32 bits of n59).  The multiply-add runs on the full-rate
- * 24-bit multiplier: it is exact modulo SCALE*2^24, and |SCALE*r| < 2^26
- * (SCALE <= 80, |r| <= 270), so sign-extending 27 bits recovers SCALE*r. */
+ * 24-bit multiplier: it is exact modulo SCALE*2^24 (a power of two >= 2^27
+ * for SCALE >= 8), and |SCALE*r| < 2^26 (SCALE <= 80, |r| <= 270), so
+ * sign-extending 27 bits recovers SCALE*r. */
 template <int P, int SCALE>
 __device__ __forceinline__ int centred_byteoff(double x, uint32_t xs) {
   static_assert(SCALE * P < (1 << 23), "i24 operand");
+  static_assert(SCALE >= 8 && (SCALE & (SCALE - 1)) == 0, "exact modulo 2^27 needs SCALE >= 8");
   constexpr double inv = 1.0 / (double)P;
   const double qd = __fma_rn(x, inv, SHIFTER);
   const int q = (int)lo32(qd);

@@ -50,6 +50,9 @@ SieveVariant pick_variant(int id) {
     case 31: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>>();
     case 32: return make_q2variant<29, 16, Levels<0, 20, 31, 51, 83>>();
     case 33: return make_q2variant<29, 16, Levels4>();
+    case 34: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 1>();
+    case 35: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 2>();
+    case 36: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 3>();
     default: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>>();
   }
 }
@@ -179,6 +182,7 @@ static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int
     a.surv_job = nullptr;
     a.job = 0;
     a.err = (unsigned int *)(ctx->d_cnt + C_ERR);
+    for (int v = 0; v < NVW; v++) a.vc[v] = vw_offset(a.shift0, VW_P[v]);
     hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
@@ -202,6 +206,7 @@ static int prepare_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, KParams *kp) {
   kp->S59 = hk.S[6];
   kp->K = K;
   kp->SHIFT = SHIFT;
+  for (int v = 0; v < NVW; v++) host_vwin(K, VW_P[v], kp->vw[v]);
   std::vector<int64_t> n43(AP26_NN43);
   host_n43(&hk, n43.data());
   HIPCHK(hipMemcpyAsync(ctx->d_n43, n43.data(), sizeof(int64_t) * AP26_NN43, hipMemcpyHostToDevice,

@@ -53,6 +53,34 @@ static inline void host_n43(const HostK *hk, int64_t *out) {
     }
 }
 
+/* Window string of a level-0 prime p computed in registers instead of read
+   from a table: with M = MOD mod p, E_p[t] = OK_p[(t*M) mod p], and bit b
+   (0..255) of out is E_p[b mod p].  For a position with residue r, bit b of
+   the masks of shifts A, A+64 (b < 128) is OK_p[(r + (A+b)*M) mod p] =
+   E_p[(r*M^-1 + A + b) mod p], a 128-bit window of out.  OK_p[v] = 0 iff
+   (p - v)*STEP^-1 mod p <= 23, or v == 0 when p divides STEP (the tables'
+   definition, src/cpu/cpusse2.cpp:31-43). */
+static inline void host_vwin(int32_t K, int p, uint64_t out[4]) {
+  const int sp = (int)(((int64_t)(K % p) * (AP26_PRIM23 % p)) % p);
+  int inv = 0;
+  if (sp) {
+    int e = p - 2, b = sp;
+    inv = 1;
+    while (e) {
+      if (e & 1) inv = inv * b % p;
+      b = b * b % p;
+      e >>= 1;
+    }
+  }
+  const int M = (int)(AP26_MOD % p);
+  for (int i = 0; i < 4; i++) out[i] = 0;
+  for (int b = 0; b < 256; b++) {
+    const int v = (b % p) * M % p;
+    const int ok = sp ? ((p - v) * inv % p > 23) : (v != 0);
+    out[b >> 6] |= (uint64_t)ok << (b & 63);
+  }
+}
+
 /* Every term the search forms for K and SHIFT fits in int64: survivors are
    below (SHIFT + 640) * MOD, and an AP with difference STEP holds at most 29
    primes in a row (29 does not divide a searched K, so one term in 29 is a

@@ -18,9 +18,14 @@
 
 namespace ap26 {
 
+/* level-0 primes whose masks can come from registers (host_vwin) */
+constexpr int NVW = 3;
+constexpr int VW_P[NVW] = {61, 67, 71};
+
 struct KParams {
   int64_t STEP, n0, S31, S37, S41, S43, S47, S53, S59;
   int32_t K, SHIFT;
+  uint64_t vw[NVW][4]; /* window strings of 61, 67, 71 (host_vwin) */
 };
 
 struct DevSol {
@@ -105,7 +110,10 @@ struct SieveArgs {
   int32_t *__restrict__ surv_job; /* batched engine: job index of each survivor */
   int32_t job;
   unsigned int *__restrict__ err;   /* set if a queue step ran on part of a wave */
+  int32_t vc[NVW];                  /* (shift0 mod p) + p*((p+1)/2), p = 61, 67, 71 */
 };
+
+__host__ __device__ constexpr int32_t vw_offset(int32_t shift0, int p) { return shift0 % p + p * ((p + 1) / 2); }
 
 /* n53 = (n43[j] + i43*S43 + i47*S47 + i53*S53) mod MOD for block
    b = j*12673 + i43*667 + i47*29 + i53, i.e. the reference's nested
@@ -546,7 +554,7 @@ sieve_persistent_kernel(PersistArgs p) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
     __syncthreads();
-    const unsigned int item = s_item;
+    const unsigned int item = __builtin_amdgcn_readfirstlane(s_item); /* uniform: keeps job data scalar */
     if (item >= nitems) break;
     const int jl = (int)(item / (AP26_NSHIFT * p.nchunk));
     const int rem = (int)(item - (unsigned int)jl * AP26_NSHIFT * p.nchunk);
@@ -686,24 +694,71 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
   return m;
 }
 
-/* level 0 of a fresh position (shift bit 0): both masks of the pair */
-template <int B, int E>
-__device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_t &m0, uint64_t &m1) {
+constexpr int pow_mod_const(int a, int e, int p) {
+  long long r = 1, b = a % p;
+  while (e) {
+    if (e & 1) r = r * b % p;
+    b = b * b % p;
+    e >>= 1;
+  }
+  return (int)r;
+}
+
+/* Both 64-bit masks of a shift pair for the prime P from its window string
+   W (host_vwin) in registers: r = n59 mod P (centred), rho = (r*M^-1 +
+   shift0) mod P by one multiply-add and a reciprocal multiply, and the
+   masks are bits [rho, rho+128) of W, by 64-bit funnel shifts. */
+template <int P>
+__device__ __forceinline__ void vwin_pair(double x, uint32_t xs1, const uint64_t *W, int C, uint32_t &a0,
+                                          uint32_t &a1, uint32_t &a2, uint32_t &a3) {
+  constexpr int U = pow_mod_const((int)(AP26_MOD % P), P - 2, P);         /* M^-1 mod P */
+  constexpr uint32_t M32 = (uint32_t)((((uint64_t)1 << 32) + P - 1) / P); /* ceil(2^32/P) */
+  const int r = centred_byteoff<P, 8>(x, xs1 * 8u) >> 3;
+  const int xv = r * U + C;                                 /* in [0, 2^13) */
+  const int q = (int)__umulhi((uint32_t)xv, M32);           /* exact: xv*P < 2^32/P */
+  const int rho = xv - q * P;                               /* [0, P) */
+  const uint32_t sh = (uint32_t)rho & 63u, ns = 63u - sh;
+  const uint64_t lo_a = (W[0] >> sh) | ((W[1] << 1) << ns); /* bits [rho, rho+64) for rho < 64 */
+  const uint64_t hi_a = (W[1] >> sh) | ((W[2] << 1) << ns);
+  uint64_t lo = lo_a, hi = hi_a;
+  if constexpr (P > 64) {
+    const uint64_t hi_b = (W[2] >> sh) | ((W[3] << 1) << ns);
+    const bool up = rho >= 64;
+    lo = up ? hi_a : lo_a;
+    hi = up ? hi_b : hi_a;
+  }
+  a0 &= (uint32_t)lo;
+  a1 &= (uint32_t)(lo >> 32);
+  a2 &= (uint32_t)hi;
+  a3 &= (uint32_t)(hi >> 32);
+}
+
+/* level 0 of a fresh position (shift bit 0): both masks of the pair; the
+   first V primes (61, 67, 71) from their window strings, the rest from LDS */
+template <int B, int E, int V = 0>
+__device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_t &m0, uint64_t &m1,
+                                            const KParams &kp, const int32_t *vc) {
+  static_assert(B == 0 && V <= NVW && V <= E, "register primes lead level 0");
   const double x = xd - TWO52;
   const uint32_t xs = lo32(xd) * 16u;
   int bo[E - B];
   uint4 w[E - B];
-  static_for<B, E>([&](auto IC) {
+  static_for<B + V, E>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     bo[I - B] = centred_byteoff<PT.p[I], 16>(x, xs);
   });
-  static_for<B, E>([&](auto IC) {
+  static_for<B + V, E>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
     w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
   });
   uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
-  static_for<B, E>([&](auto IC) {
+  static_for<0, V>([&](auto IC) {
+    constexpr int I = decltype(IC)::value;
+    static_assert(PT.p[I] == VW_P[I], "register primes are 61, 67, 71");
+    vwin_pair<PT.p[I]>(x, lo32(xd), kp.vw[I], vc[I], a0, a1, a2, a3);
+  });
+  static_for<B + V, E>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     a0 &= w[I - B].x;
     a1 &= w[I - B].y;
@@ -838,7 +893,7 @@ struct QSieve2 {
   }
 };
 
-template <int NL, class LV>
+template <int NL, class LV, int V = 0>
 __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride) {
   const double S59d = (double)a.kp.S59;
@@ -852,7 +907,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
     double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
     for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
       uint64_t m0, m1;
-      level0_pair<0, LV::B[1]>(xd, qs.ldsb, m0, m1);
+      level0_pair<0, LV::B[1], V>(xd, qs.ldsb, m0, m1, a.kp, a.vc);
       m0 = has ? m0 : 0;
       m1 = has ? m1 : 0;
       qs.push(0, xd, m0, m0 != 0);
@@ -877,8 +932,9 @@ constexpr int qsieve2_lds_bytes() {
   return 2 * (PT.off[NL] + 90) * 8 + WAVES * (LV::N - 1) * 128 * (int)sizeof(QEntry);
 }
 
-/* Items (job, shift pair, n43 chunk); tables [entry][shift in pair]. */
-template <int NL, int WAVES, class LV>
+/* Items (job, shift pair, n43 chunk); tables [entry][shift in pair].  V: the
+   first V level-0 primes from registers (vwin_pair) instead of LDS. */
+template <int NL, int WAVES, class LV, int V = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_persistent2_kernel(PersistArgs p) {
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
@@ -898,7 +954,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
     __syncthreads();
-    const unsigned int item = s_item;
+    const unsigned int item = __builtin_amdgcn_readfirstlane(s_item); /* uniform: keeps job data scalar */
     if (item >= nitems) {
       close_survivors2(sc, p.surv, p.cap, threadIdx.x & 63);
       break;
@@ -923,6 +979,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     a.tab = tab;
     a.shift0 = job.kp.SHIFT + 128 * t;
     a.nslot = 2;
+    for (int v = 0; v < NVW; v++) a.vc[v] = vw_offset(a.shift0, VW_P[v]);
     const int j0 = chunk * p.chunk;
     const int j1 = min(AP26_NN43, j0 + p.chunk);
     a.blk0 = (int64_t)j0 * AP26_N53_PER_N43;
@@ -942,14 +999,14 @@ sieve_persistent2_kernel(PersistArgs p) {
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV>(qs, a, threadIdx.x, blockDim.x);
+    qsieve2_blocks<NL, LV, V>(qs, a, threadIdx.x, blockDim.x);
   }
 }
 
 /* One launch per (K, shift pair): the plain path with the shift-pair sieve
    (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
    both shifts), survivors in per-wave slot chunks as in the batched engine. */
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, int V = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_queue2_kernel(SieveArgs a) {
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
@@ -971,7 +1028,8 @@ sieve_queue2_kernel(SieveArgs a) {
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, (int64_t)gridDim.x * blockDim.x);
+  qsieve2_blocks<NL, LV, V>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                            (int64_t)gridDim.x * blockDim.x);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }
 
