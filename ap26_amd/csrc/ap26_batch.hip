@@ -37,9 +37,9 @@ PersistVariant make_pvariant() {
   return PersistVariant{NL, WAVES, 1, &sieve_persistent_kernel<NL, WAVES, LV>};
 }
 
-template <int NL, int WAVES, class LV, int V = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0>
 PersistVariant make_pvariant2() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, V>};
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, V, G>};
 }
 
 /* 1..3, 9: one shift per item; 0 (default = 28), 10..34: shift pairs.  Level 0
@@ -81,6 +81,8 @@ PersistVariant pick_pvariant(int id) {
     case 36: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 2>();
     case 37: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 3>();
     case 38: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>, 3>();
+    case 39: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 2>();
+    case 40: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 4>();
     default: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
   }
 }

@@ -734,10 +734,12 @@ __device__ __forceinline__ void vwin_pair(double x, uint32_t xs1, const uint64_t
 }
 
 /* level 0 of a fresh position (shift bit 0): both masks of the pair; the
-   first V primes (61, 67, 71) from their window strings, the rest from LDS */
-template <int B, int E, int V = 0>
-__device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_t &m0, uint64_t &m1,
-                                            const KParams &kp, const int32_t *vc) {
+   first V primes (61, 67, 71) from their window strings, the last G through
+   the vector-memory path (L1; same table in global memory), the rest from
+   LDS: the three paths run in parallel, and LDS is the busiest. */
+template <int B, int E, int V = 0, int G = 0>
+__device__ __forceinline__ void level0_pair(double xd, const char *ldsb, const char *gtb, uint64_t &m0,
+                                            uint64_t &m1, const KParams &kp, const int32_t *vc) {
   static_assert(B == 0 && V <= NVW && V <= E, "register primes lead level 0");
   const double x = xd - TWO52;
   const uint32_t xs = lo32(xd) * 16u;
@@ -750,7 +752,10 @@ __device__ __forceinline__ void level0_pair(double xd, const char *ldsb, uint64_
   static_for<B + V, E>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
-    w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
+    if constexpr (I >= E - G)
+      w[I - B] = *(const uint4 *)(gtb + (int64_t)(C0 + bo[I - B]));
+    else
+      w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
   });
   uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
   static_for<0, V>([&](auto IC) {
@@ -893,7 +898,7 @@ struct QSieve2 {
   }
 };
 
-template <int NL, class LV, int V = 0>
+template <int NL, class LV, int V = 0, int G = 0>
 __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride) {
   const double S59d = (double)a.kp.S59;
@@ -907,7 +912,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
     double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
     for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
       uint64_t m0, m1;
-      level0_pair<0, LV::B[1], V>(xd, qs.ldsb, m0, m1, a.kp, a.vc);
+      level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
       m0 = has ? m0 : 0;
       m1 = has ? m1 : 0;
       qs.push(0, xd, m0, m0 != 0);
@@ -934,7 +939,7 @@ constexpr int qsieve2_lds_bytes() {
 
 /* Items (job, shift pair, n43 chunk); tables [entry][shift in pair].  V: the
    first V level-0 primes from registers (vwin_pair) instead of LDS. */
-template <int NL, int WAVES, class LV, int V = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_persistent2_kernel(PersistArgs p) {
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
@@ -999,14 +1004,14 @@ sieve_persistent2_kernel(PersistArgs p) {
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V>(qs, a, threadIdx.x, blockDim.x);
+    qsieve2_blocks<NL, LV, V, G>(qs, a, threadIdx.x, blockDim.x);
   }
 }
 
 /* One launch per (K, shift pair): the plain path with the shift-pair sieve
    (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
    both shifts), survivors in per-wave slot chunks as in the batched engine. */
-template <int NL, int WAVES, class LV, int V = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_queue2_kernel(SieveArgs a) {
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
@@ -1028,7 +1033,7 @@ sieve_queue2_kernel(SieveArgs a) {
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV, V>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+  qsieve2_blocks<NL, LV, V, G>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
                             (int64_t)gridDim.x * blockDim.x);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }

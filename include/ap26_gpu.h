@@ -54,7 +54,7 @@ typedef struct {
 typedef struct {
   int32_t sieve_variant;    /* 0 default (shift-pair queued-compaction sieve),
                                1 simple per-shift sieve, 2 simple sieve with 2
-                               shifts per pass, 10..33 tuning candidates */
+                               shifts per pass, 10..36 tuning candidates */
   int32_t block;            /* (reported) threads per workgroup of the sieve */
   int32_t blocks_per_cu;    /* resident workgroups per CU (grid = CUs x this) */
   int32_t verbose;
