@@ -9,7 +9,10 @@
  *   -report N      SOL threshold, the reference's MINIMUM_AP_LENGTH_TO_REPORT (20)
  *   -shiftckpt     checkpoint after every shift (ITER = shifts done), as the
  *                  reference CPU build does; default is one checkpoint per K
- *   -batch         search the whole unit with the batched persistent engine
+ *   -batch         search the unit with the batched persistent engine, 256
+ *                  searched K per call (AP26_BATCH_K_PER_CALL overrides); the
+ *                  state files, records and hash line are those of the per-K
+ *                  run, one checkpoint per K
  *   -q             no Solution/Checkpoint lines on stdout
  *   -metrics FILE  append a progress record (progress, cputime, runtime, K,
  *                  ITER, counts: the trickle-up fields as metrics) to FILE at
@@ -36,6 +39,8 @@
 #include "ap26_host.h"
 
 #define SOL_CAP 65536
+#define BATCH_K_PER_CALL 256     /* -batch: searched K per batched call */
+#define BATCH_SOL_CAP (1 << 20)  /* -batch: report buffer per call */
 
 static int cmp_sol(const void *a, const void *b) {
   const ap26_sol *x = (const ap26_sol *)a, *y = (const ap26_sol *)b;
@@ -308,26 +313,66 @@ int main(int argc, char *argv[]) {
   printf("%d K in this range remaining to be searched (%d skipped, %d done).\n", h.K_COUNT - h.K_DONE,
          KMAX + 1 - KMIN - h.K_COUNT, h.K_DONE);
 
-  ap26_sol *sols = (ap26_sol *)malloc(sizeof(ap26_sol) * SOL_CAP);
+  ap26_sol *sols = (ap26_sol *)malloc(sizeof(ap26_sol) * (batch ? BATCH_SOL_CAP : SOL_CAP));
   int status = EXIT_SUCCESS;
   if (!sols) return EXIT_FAILURE;
 
-  if (batch && K <= KMAX) {
-    /* one unit: the remaining range, resumed shift for its first K */
-    ap26_range r = {K, KMAX, SHIFT, ITER};
-    int32_t n = 0;
-    int64_t tot = 0;
-    ap26h_checkpoint(&h, K, 0, ITER);
-    rc = ap26_search_batch(ctx, &r, 1, sols, SOL_CAP, &n, &tot, NULL);
-    if (rc) {
-      fprintf(stderr, "Error: GPU search failed: %s\n", ap26_strerror(rc));
-      status = EXIT_FAILURE;
-    } else if ((rc = report_all(&h, sols, n)) != 0) {
-      status = EXIT_FAILURE;
-    } else {
-      K = KMAX + 1;
-      ITER = 0;
-      h.K_DONE = h.K_COUNT;
+  if (batch) {
+    /* The batched engine on chunks of up to BATCH_K_PER_CALL searched K (the
+       first resumed at ITER).  Each chunk's reports are replayed K by K as
+       the per-K loop below writes them (checkpoint before the K, its reports
+       sorted by first term, K_DONE), so the state files advance per K and an
+       interrupted run resumes at the K after the last replayed one.  A chunk
+       whose reports exceed the buffer is searched again in halves. */
+    int32_t per_call = BATCH_K_PER_CALL;
+    const char *env = getenv("AP26_BATCH_K_PER_CALL");
+    if (env && atoi(env) > 0) per_call = atoi(env);
+    while (K <= KMAX && status == EXIT_SUCCESS) {
+      int cnt = 0;
+      int32_t Kend = K - 1;
+      for (int32_t k2 = K; k2 <= KMAX && cnt < per_call; k2++)
+        if (ap26h_will_search(k2)) {
+          cnt++;
+          Kend = k2;
+        }
+      if (cnt == 0) {
+        K = KMAX + 1;
+        ITER = 0;
+        break;
+      }
+      ap26_range r = {K, Kend, SHIFT, ITER};
+      int32_t n = 0;
+      rc = ap26_search_batch(ctx, &r, 1, sols, BATCH_SOL_CAP, &n, NULL, NULL);
+      if (rc == AP26_E_OVERFLOW && per_call > 1) {
+        per_call = (per_call + 1) / 2;
+        continue;
+      }
+      if (rc) {
+        fprintf(stderr, "Error: GPU search failed: %s\n", ap26_strerror(rc));
+        status = EXIT_FAILURE;
+        break;
+      }
+      qsort(sols, (size_t)n, sizeof(ap26_sol), cmp_sol_unit);
+      int32_t s = 0;
+      for (int32_t k = K; k <= Kend && status == EXIT_SUCCESS; k++) {
+        if (!ap26h_will_search(k)) continue;
+        ap26h_checkpoint(&h, k, 0, ITER);
+        for (; s < n && sols[s].K == k; s++) {
+          rc = ap26h_report_masks(&h, sols[s].k, sols[s].K, sols[s].first, sols[s].prp2_mask,
+                                  sols[s].valid_mask);
+          if (rc) {
+            status = EXIT_FAILURE;
+            break;
+          }
+        }
+        ITER = 0;
+        if (status == EXIT_SUCCESS) h.K_DONE++;
+      }
+      if (status != EXIT_SUCCESS) {
+        fprintf(stderr, "Error: reporting for K %d..%d failed\n", K, Kend);
+        break;
+      }
+      K = Kend + 1;
     }
   } else {
     for (; K <= KMAX && status == EXIT_SUCCESS; ++K) {

@@ -12,8 +12,9 @@ pytestmark = pytest.mark.gpu
 AP26 = os.path.join(ROOT, "ap26_amd", "bin", "AP26")
 
 
-def run(args, cwd):
-    r = subprocess.run([AP26, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300)
+def run(args, cwd, env=None):
+    full = None if env is None else {**os.environ, **env}
+    r = subprocess.run([AP26, *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=300, env=full)
     return r
 
 
@@ -105,6 +106,39 @@ def test_batch_flag_same_output(golden_index, tmp_path):
     assert r.returncode == 0, r.stderr
     want = _units_hash(golden_index, ["c1_875859", "c1_875860"], 875858, 875860)
     assert read_sol(str(tmp_path / "SOL-AP26.txt")) == want
+
+
+def test_batch_flag_chunks_equal_per_k_run(tmp_path):
+    """`-batch` runs the unit in chunks of searched K (here one K per call) and
+    replays each K's checkpoint and reports: SOL file and state files are byte
+    for byte those of the per-K run, also when resuming mid-unit."""
+    unit = [875858, 875862, 640, "-report", 10]
+    outs = {}
+    for name, extra, env in (("perk", [], None), ("batch1", ["-batch"], {"AP26_BATCH_K_PER_CALL": "1"}),
+                             ("batch2", ["-batch"], {"AP26_BATCH_K_PER_CALL": "2"})):
+        d = tmp_path / name
+        d.mkdir()
+        r = run(unit + extra, d, env)
+        assert r.returncode == 0, (name, r.stderr)
+        outs[name] = ((d / "SOL-AP26.txt").read_bytes(), _state_fields(d))
+    assert outs["batch1"] == outs["perk"]
+    assert outs["batch2"] == outs["perk"]
+    # resume at K=875861 from a checkpoint written by the per-K run's sequence
+    sol = outs["perk"][0].decode().splitlines()
+    T = 0
+    for line in sol[:-1]:
+        k, K, first = (int(x) for x in line.split())
+        if K < 875861:
+            T += (first % 1000) + k
+    res = {}
+    for name, extra, env in (("perk", [], None), ("batch", ["-batch"], {"AP26_BATCH_K_PER_CALL": "1"})):
+        d = tmp_path / ("resume_" + name)
+        d.mkdir()
+        (d / "AP26-state.a.txt").write_text(f"875858 875862 640 875861 0 {T} 1234.000000\n")
+        r = run(unit + extra, d, env)
+        assert r.returncode == 0, (name, r.stderr)
+        res[name] = ((d / "SOL-AP26.txt").read_bytes(), _state_fields(d))
+    assert res["batch"] == res["perk"]
 
 
 def _state_fields(d):
