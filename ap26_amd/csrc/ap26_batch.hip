@@ -9,6 +9,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -23,6 +24,7 @@ using namespace ap26;
 namespace {
 
 constexpr int BATCH_SOL_CAP = 1 << 20;
+constexpr int BATCH_SEG_K = 2048; /* searched K per segment of one batched call */
 constexpr int MAX_RANGES = 1 << 20;
 
 using PersistFn = void (*)(PersistArgs);
@@ -385,64 +387,103 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   int rc = batch_init(ctx);
   if (rc) return rc;
   BatchState *b = ctx->batch;
-  rc = ensure_capacity(b, (int)jobs.size(), nr);
+  /* Segments of at most seg_k searched K run back to back; each ends with
+     the device verification and one copy of its reports, so the device
+     report buffer (BATCH_SOL_CAP, ~180 reports per K) bounds a segment, not
+     the call.  A segment whose reports overflow it runs again in halves. */
+  int seg_k = BATCH_SEG_K;
+  unsigned int sol_limit = BATCH_SOL_CAP;
+  {
+    /* test hooks: a smaller segment, a smaller report budget per segment */
+    const char *e = getenv("AP26_BATCH_SEG_K");
+    if (e && atoi(e) > 0) seg_k = atoi(e);
+    e = getenv("AP26_BATCH_SOL_LIMIT");
+    if (e && atoi(e) > 0 && atoi(e) < BATCH_SOL_CAP) sol_limit = (unsigned int)atoi(e);
+  }
+  rc = ensure_capacity(b, (int)std::min<size_t>(jobs.size(), (size_t)seg_k), nr);
   if (rc) return rc;
-  /* survivor slots: the heaviest round's budget (K divisible by sieve
-     primes keep more survivors) */
-  int64_t need = 0;
-  for (size_t r0 = 0; r0 < jobs.size(); r0 += (size_t)b->per_round) {
-    int64_t sum = 0;
-    for (size_t j = r0; j < jobs.size() && j < r0 + (size_t)b->per_round; j++) sum += host_surv_budget(jobs[j].kp.K);
-    need = sum > need ? sum : need;
-  }
-  need += b->surv_slack;
-  const int rounds = (int)((jobs.size() + b->per_round - 1) / b->per_round);
-  const int timed = rounds < 1024 ? rounds : 1024;
-  while ((int)b->ev.size() < 2 * timed) {
-    hipEvent_t e;
-    HIPCHK(hipEventCreate(&e));
-    b->ev.push_back(e);
-  }
+  std::vector<DevSol> hs;
+  std::vector<unsigned long long> tot_all(nr, 0), tot(nr);
   unsigned long long cnt[8];
-  std::vector<unsigned long long> tot(nr);
-  for (int attempt = 0;; attempt++) {
-    rc = ensure_surv_cap(ctx, b, need);
-    if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(b->d_jobs, jobs.data(), sizeof(BatchJobDev) * jobs.size(), hipMemcpyHostToDevice,
-                          ctx->stream));
-    const int nres = nr > 8 ? nr : 8;
-    hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs,
-                       (int)jobs.size(), b->per_round, b->d_cnt, b->d_hash, nr);
-    HIPCHK(hipGetLastError());
-    rc = ensure_graph(ctx, b);
-    if (rc) return rc;
-    HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
-    for (int i = 0; i < rounds; i++) {
-      rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1]) : launch_round(ctx, b, nullptr, nullptr);
-      if (rc) return rc;
+  int timed_total = 0;
+  for (size_t seg0 = 0; seg0 < jobs.size();) {
+    const int segn = (int)std::min<size_t>(jobs.size() - seg0, (size_t)seg_k);
+    const BatchJobDev *sj = jobs.data() + seg0;
+    /* survivor slots: the heaviest round's budget (K divisible by sieve
+       primes keep more survivors) */
+    int64_t need = 0;
+    for (int r0 = 0; r0 < segn; r0 += b->per_round) {
+      int64_t sum = 0;
+      for (int j = r0; j < segn && j < r0 + b->per_round; j++) sum += host_surv_budget(sj[j].kp.K);
+      need = sum > need ? sum : need;
     }
-    hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream,
-                       b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
-    hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream,
-                       b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
-    HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    if (cnt[6]) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
-    /* a round kept more survivors than its budget: grow the buffers to the
-       largest count seen and run the batch again */
-    if (cnt[4] == 0) break;
-    if (attempt) return AP26_E_INTERNAL;
-    need = (int64_t)(cnt[4] + (cnt[4] >> 3)) + b->surv_slack;
+    need += b->surv_slack;
+    const int rounds = (segn + b->per_round - 1) / b->per_round;
+    const int timed = rounds < 1024 ? rounds : 1024;
+    while ((int)b->ev.size() < 2 * timed) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      b->ev.push_back(e);
+    }
+    for (int attempt = 0;; attempt++) {
+      rc = ensure_surv_cap(ctx, b, need);
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(b->d_jobs, sj, sizeof(BatchJobDev) * segn, hipMemcpyHostToDevice, ctx->stream));
+      const int nres = nr > 8 ? nr : 8;
+      hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs, segn,
+                         b->per_round, b->d_cnt, b->d_hash, nr);
+      HIPCHK(hipGetLastError());
+      rc = ensure_graph(ctx, b);
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
+      for (int i = 0; i < rounds; i++) {
+        rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1])
+                       : launch_round(ctx, b, nullptr, nullptr);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, b->d_sols,
+                         (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+      hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, b->d_sols,
+                         (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
+      HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
+                            ctx->stream));
+      HIPCHK(hipStreamSynchronize(ctx->stream));
+      if (cnt[6]) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
+      /* a round kept more survivors than its budget: grow the buffers to the
+         largest count seen and run the segment again */
+      if (cnt[4] == 0) break;
+      if (attempt) return AP26_E_INTERNAL;
+      need = (int64_t)(cnt[4] + (cnt[4] >> 3)) + b->surv_slack;
+    }
+    const unsigned int nsol = (unsigned int)(cnt[2] & 0xffffffffu);
+    if (nsol > sol_limit && segn > 1) {
+      seg_k = (segn + 1) / 2;
+      continue;
+    }
+    if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL; /* one K, > 2^20 reports */
+    const size_t h0 = hs.size();
+    hs.resize(h0 + nsol);
+    if (nsol) HIPCHK(hipMemcpy(hs.data() + h0, b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+    for (int i = 0; i < nr; i++) tot_all[i] += tot[i];
+    st.prp_tests += (int64_t)cnt[1];
+    st.survivors += (int64_t)cnt[5];
+    {
+      /* ms_walk: device time of all rounds; ms_sieve: sum over the timed
+         rounds of the sieve graph (n43 starts + tables + persistent sieve) */
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) st.ms_walk += ms;
+      for (int i = 0; i < timed; i++)
+        if (hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]) == hipSuccess) st.ms_sieve += ms;
+    }
+    timed_total += timed;
+    seg0 += (size_t)segn;
   }
-  const unsigned int nsol = (unsigned int)(cnt[2] & 0xffffffffu);
-  if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
-  std::vector<DevSol> hs(nsol);
-  if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
-  const int ncopy = (int)nsol < cap ? (int)nsol : cap;
+  const int64_t nsol = (int64_t)hs.size();
+  if (nsol > INT32_MAX) return AP26_E_INTERNAL;
+  const int ncopy = nsol < cap ? (int)nsol : cap;
   for (int i = 0; i < ncopy; i++) {
     out[i].k = hs[i].k;
     out[i].K = hs[i].K;
@@ -454,24 +495,14 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   }
   *n_out = (int32_t)nsol;
   if (hash_totals)
-    for (int i = 0; i < nr; i++) hash_totals[i] = (int64_t)tot[i];
-  st.prp_tests = (int64_t)cnt[1];
-  st.survivors = (int64_t)cnt[5];
+    for (int i = 0; i < nr; i++) hash_totals[i] = (int64_t)tot_all[i];
   st.reports = nsol;
   for (const BatchJobDev &j : jobs)
     st.positions += (int64_t)AP26_NN43 * AP26_N53_PER_N43 * AP26_N59_STEPS * (AP26_NSHIFT - j.it0);
-  {
-    /* ms_walk: device time of all rounds; ms_sieve: sum over the timed rounds
-       of the sieve graph (n43 starts + tables + persistent sieve) */
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]) == hipSuccess) st.ms_walk = ms;
-    for (int i = 0; i < timed; i++)
-      if (hipEventElapsedTime(&ms, b->ev[2 * i], b->ev[2 * i + 1]) == hipSuccess) st.ms_sieve += ms;
-  }
-  st.sieve_launches = timed;
+  st.sieve_launches = timed_total;
   st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
-  return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+  return nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
 }
 
 extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, int64_t *out,

@@ -125,9 +125,13 @@ int ap26_prp_batch(ap26_ctx *ctx, const uint64_t *n, uint8_t *out, size_t count)
 int ap26_prp_batch_dev(ap26_ctx *ctx, const void *d_n, void *d_out, size_t count, double *ms);
 
 /* Many work units in one persistent launch sequence (work-stealing K queue,
-   captured hipGraph).  out gets every report with range_id = index into r;
-   hash_totals[i] = exact sum over range i of (first % 1000 + k) for the raw
-   reports (the host applies the reference's cap and Non-Solution rules). */
+   captured hipGraph per round of 16 K).  out gets every report with range_id =
+   index into r; hash_totals[i] = exact sum over range i of (first % 1000 + k)
+   for the raw reports (the host applies the reference's cap and Non-Solution
+   rules).  The searched K run in segments of 2048 (device verification and
+   one report copy per segment), so a call may hold any number of K; *n_out =
+   total reports, AP26_E_OVERFLOW if > cap (the search is complete; call
+   again with a larger buffer). */
 int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out,
                       int32_t cap, int32_t *n_out, int64_t *hash_totals, ap26_stats *stats);
 

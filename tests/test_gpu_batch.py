@@ -61,6 +61,27 @@ def test_batch_many_rounds_against_golden(gpu_ctx, golden_index):
         assert got == recs
 
 
+def test_batch_segments_equal_one_segment(gpu_ctx, golden_index, monkeypatch):
+    """A call runs its K in segments (default 2048 searched K), each with its
+    own verification and report copy, and halves a segment whose reports
+    exceed the budget: reports, per-range totals and counts equal a
+    one-segment call.  Here segments of 5 K and a budget of 400 reports
+    (about 2 K) force both the split and the halving."""
+    units = [g for g in golden_index if g["kmin"] == g["kmax"]][:12]
+    ranges = [(g["kmin"], g["kmax"], g["shift"], 0) for g in units]
+    ranges.append((875859, 875862, 640, 2))  # a multi-K range resumed at ITER 2
+    sols0, tot0, st0 = gpu_ctx.search_batch(ranges)
+    monkeypatch.setenv("AP26_BATCH_SEG_K", "5")
+    monkeypatch.setenv("AP26_BATCH_SOL_LIMIT", "400")
+    sols1, tot1, st1 = gpu_ctx.search_batch(ranges)
+    for i in range(len(ranges)):
+        assert key([x for x in sols1 if x["range_id"] == i]) == key([x for x in sols0 if x["range_id"] == i]), i
+    assert list(tot1) == list(tot0)
+    d0, d1 = st0.as_dict(), st1.as_dict()
+    for f in ("k_searched", "positions", "survivors", "prp_tests", "reports"):
+        assert d1[f] == d0[f], f
+
+
 @pytest.mark.parametrize("variant", ["0", "10", "9", "12", "13", "32"])
 def test_batch_sieve_matches_plain_survivors(variant, monkeypatch):
     """The batched sieve (shift-pair and one-shift persistent kernels) keeps
