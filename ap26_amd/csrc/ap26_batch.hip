@@ -47,7 +47,9 @@ PersistVariant make_pvariant2() {
 /* 1..3, 9: one shift per item; 0 (default = 28), 10..34: shift pairs.  Level 0
    (every position, both shifts of a pair per 16-byte read) runs on the first
    18..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
-   so level 0 pays off further than on one shift (61..131). */
+   so level 0 pays off further than on one shift (61..131).  35..38: the first
+   1..3 level-0 primes from register bit windows; 39..40: the last 2 or 4
+   level-0 primes read through L1 instead of LDS. */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();

@@ -21,7 +21,8 @@ using namespace ap26;
 /* Sieve variants: 0 = default (31: shift-pair queued sieve, level 0 on 18
    primes), 1 = simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve
    tuning candidates (LDS primes, waves per CU, levels, VGPR constants),
-   31..33 = shift-pair queued sieves. */
+   31..33 = shift-pair queued sieves, 34..36 = shift-pair with the first 1..3
+   level-0 primes from register bit windows instead of LDS. */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
