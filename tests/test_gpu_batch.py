@@ -105,6 +105,12 @@ def test_batch_edge_cases(gpu_ctx):
         gpu_ctx.search_batch([(5, 4, 0, 0), (0, 3, 0, 0)])
     with pytest.raises(gpu.Ap26Error):  # terms past 2^63
         gpu_ctx.search_batch([(366384, 366384, 36000, 0)])
+    # the largest K and SHIFT the bound admits (terms up to 8.7e18 < 2^63):
+    # batched equals plain, survivors and reports
+    sols, tot, st = gpu_ctx.search_batch([(600000000, 600000000, 16912, 0)])
+    psols, pst = gpu_ctx.search_k(600000000, 16912)
+    assert st.survivors == pst.survivors > 30_000_000
+    assert key(sols) == key(psols)
 
 
 def test_server_alone_and_in_mix(gpu_ctx):

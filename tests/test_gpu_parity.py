@@ -49,7 +49,8 @@ def test_prp_microbench_distribution_sample(gpu_ctx):
 SPANS = [(366384, 0, 0, 10, 0, 3), (875859, 640, 0, 10, 100, 102), (48620635, 640, 3, 7, 5000, 5002),
          (44121555, 0, 0, 10, 10838, 10840), (976061, 0, 2, 3, 7000, 7004),
          (99999989, 1280, 9, 10, 1, 2), (1000003, 1280, 0, 10, 5419, 5421),
-         (61 * 67 * 71 * 73, 0, 0, 10, 3000, 3002)]
+         (61 * 67 * 71 * 73, 0, 0, 10, 3000, 3002),
+         (600000000, 16912, 0, 10, 4000, 4002)]  # largest K and SHIFT: terms just below 2^63
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 23, 32])
@@ -62,7 +63,8 @@ def test_sieve_span_parity(variant, span):
 
 
 @pytest.mark.parametrize("span", [(366384, 0, 0, 10, 0, 20), (875862, 640, 0, 10, 2000, 2020),
-                                  (44121555, 0, 0, 10, 10800, 10840), (61 * 67 * 71 * 73, 0, 0, 10, 100, 110)])
+                                  (44121555, 0, 0, 10, 10800, 10840), (61 * 67 * 71 * 73, 0, 0, 10, 100, 110),
+                                  (600000000, 16912, 0, 10, 0, 20)])
 def test_search_span_parity(gpu_ctx, span):
     g, st = gpu_ctx.search_span(*span)
     o, nsurv = O.search_span(*span, threads=4)
