@@ -33,7 +33,9 @@ DST = os.path.join(HERE, "ref_min10")
 # at both ends of the batched K range; p61_*: a K divisible by 61, where
 # STEP = 0 (mod 61); h4_*: K = 61*67*71*73, the most sieve primes a searched
 # K <= 6e8 can hold, so the sieve keeps about 5.4x the usual survivors;
-# range_*: a multi-K unit with a skipped K.
+# range_*: a multi-K unit with a skipped K; k6e8_*: the largest K the build
+# admits (6e8, where STEP = K*23# is largest) at SHIFT 640 and 1280; mid_*: a
+# K between the config ranges and 6e8; s5120_*: a SHIFT beyond the configs'.
 RUN_LIST = [
     ("t366384", 366384, 366384, 0),
     ("t44121555", 44121555, 44121555, 0),
@@ -47,6 +49,10 @@ RUN_LIST = [
     ("p61_976061", 976061, 976061, 0),
     ("range_875858_875860", 875858, 875860, 640),
     ("h4_21182921", 21182921, 21182921, 0),
+    ("k6e8_600000000", 600000000, 600000000, 640),
+    ("k6e8_599999999", 599999999, 599999999, 1280),
+    ("mid_300000007", 300000007, 300000007, 0),
+    ("s5120_1000003", 1000003, 1000003, 5120),
 ]
 
 
