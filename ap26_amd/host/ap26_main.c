@@ -149,14 +149,6 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
     x->h.quiet = quiet;
     unit_begin(&x->h, &x->K, &x->ITER);
     ap26h_count(&x->h, x->K);
-    if (metrics) {
-      char mp[1024];
-      snprintf(mp, sizeof(mp), "%s/%s", dir, metrics);
-      if (ap26h_open_metrics(&x->h, mp)) {
-        status = EXIT_FAILURE;
-        break;
-      }
-    }
   }
   fclose(f);
   ap26_sol *sols = status == EXIT_SUCCESS ? (ap26_sol *)malloc(sizeof(ap26_sol) * MANIFEST_SOL_CAP) : NULL;
@@ -177,6 +169,16 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
           last = k2;
         }
       x->Kend = cnt ? last : x->h.KMAX;
+      if (metrics && !x->h.metrics_file) {
+        /* opened when the unit starts and closed when it ends, so a long
+           manifest holds files only for the units in flight */
+        char mp[1024];
+        snprintf(mp, sizeof(mp), "unit_%d_%d_%d/%s", x->h.KMIN, x->h.KMAX, x->h.SHIFT, metrics);
+        if (ap26h_open_metrics(&x->h, mp)) {
+          status = EXIT_FAILURE;
+          break;
+        }
+      }
       if (cnt) {
         r[nr] = (ap26_range){x->K, x->Kend, x->h.SHIFT, x->ITER};
         owner[nr++] = i;
@@ -186,7 +188,7 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
         r[nr++] = (ap26_range){x->K, x->K - 1, x->h.SHIFT, 0};
       }
     }
-    if (nr == 0) break;
+    if (status != EXIT_SUCCESS || nr == 0) break;
     int32_t n = 0;
     int rc = ap26_search_batch(ctx, r, nr, sols, MANIFEST_SOL_CAP, &n, NULL, NULL);
     if (rc) {
