@@ -43,6 +43,12 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
+/* Number of set bits of the wave mask bal in the lanes below this one
+   (v_mbcnt_lo / v_mbcnt_hi): a lane's slot among the lanes that append. */
+__device__ __forceinline__ int lanes_below(uint64_t bal) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+}
+
 /* ---- table build ------------------------------------------------------- */
 /* OK_p[v] is 0 iff some term n + i*STEP (i in 0..23) is divisible by p when
  * n == v (mod p) (src/cpu/cpusse2.cpp:31-35): v == -i*STEP.  With sp = STEP mod p
@@ -797,7 +803,7 @@ __device__ __forceinline__ void emit_survivors2(uint64_t w, double xd, const Sie
     if (bal == 0) break;
     const int cnt = __popcll(bal);
     const int leader = __ffsll((unsigned long long)bal) - 1;
-    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
+    const int pre = lanes_below(bal);
     unsigned long long fresh = 0;
     if (cnt > cur.left) { /* wave-uniform */
       if (lane == leader) fresh = atomicAdd(a.nsurv, (unsigned long long)SURV_CHUNK);
@@ -837,21 +843,25 @@ struct QSieve2 {
   static constexpr int NQ = LV::N - 1;
   QEntry (*q)[QCAP];
   int qn[NQ];
-  bool partial; /* a push ran with lanes switched off (must never happen) */
+  bool partial; /* a queue step ran with lanes switched off (must never happen) */
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
   SurvCursor *cur;
 
+  /* Every lane keeps the same counts only if every lane runs every push.
+     Pushes happen in straight-line code of the position steps, drain
+     iterations and flushes, whose loops are uniform; each of those checks
+     once that the whole wave runs it, and a failure is reported (the queue
+     would lose entries). */
+  __device__ __forceinline__ void check_full_wave() {
+    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
+  }
+
   __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
-    const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
-    if (pred) q[qi][qn[qi] + pre] = QEntry{xd, m};
-    /* every lane keeps the same count only if every lane runs every push:
-       the callers' loops are uniform, and a push on part of the wave is
-       reported (the queue would lose entries) */
+    if (pred) q[qi][qn[qi] + lanes_below(bal)] = QEntry{xd, m};
     qn[qi] += __popcll(bal);
-    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -871,6 +881,7 @@ struct QSieve2 {
   template <int L>
   __device__ __forceinline__ void drain() {
     while (qn[L] >= 64) {
+      check_full_wave();
       qn[L] -= 64;
       const QEntry e = q[L][qn[L] + lane];
       __builtin_amdgcn_wave_barrier();
@@ -884,6 +895,7 @@ struct QSieve2 {
       drain<L>();
       const int n = qn[L];
       if (n > 0) {
+        check_full_wave();
         const bool pred = lane < n;
         QEntry e{TWO52, 0};
         if (pred) e = q[L][lane];
@@ -911,6 +923,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
     if (__ballot(has) == 0) break;
     double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
     for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
+      qs.check_full_wave();
       uint64_t m0, m1;
       level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
       m0 = has ? m0 : 0;
