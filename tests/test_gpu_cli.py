@@ -9,7 +9,7 @@ from ap26_amd import host as H
 from conftest import GOLDEN, ROOT, read_sol
 
 pytestmark = pytest.mark.gpu
-AP26 = os.path.join(ROOT, "ap26_amd", "bin", "AP26")
+AP26 = os.environ.get("AP26_BIN", os.path.join(ROOT, "ap26_amd", "bin", "AP26"))
 
 
 def run(args, cwd, env=None):
