@@ -46,13 +46,27 @@ def _atomic_link(tmp, dst):
 
 
 def build_gpu(verbose=False):
+    """Compile the HIP sources to objects in parallel, then link the library."""
     srcs = [os.path.join(CSRC, "ap26_gpu.hip"), os.path.join(CSRC, "ap26_batch.hip")]
     srcs = [s for s in srcs if os.path.exists(s)]
-    tmp = LIB_GPU + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *srcs]
+    objs = [os.path.join(PKG, os.path.basename(s) + ".o") for s in srcs]
+    cmds = [[HIPCC, *HIP_FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
     if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
+        for c in cmds:
+            print(" ".join(c))
+    procs = [subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for c in cmds]
+    failed = []
+    for c, pr in zip(cmds, procs):
+        out, err = pr.communicate()
+        if pr.returncode != 0:
+            sys.stderr.write(out + err)
+            failed.append(" ".join(c))
+    if failed:
+        raise RuntimeError("build step failed: " + "; ".join(failed))
+    tmp = LIB_GPU + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
+    for o in objs:
+        os.remove(o)
     _atomic_link(tmp, LIB_GPU)
     return LIB_GPU
 
