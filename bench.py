@@ -272,7 +272,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the search shards by K with no inter-GPU traffic; see DESIGN.md)", file=sys.stderr)
+              "(the search shards by K with no inter-GPU traffic: each GPU would run its own units, "
+              "and only the reports and per-unit hash totals would be gathered)", file=sys.stderr)
         sys.exit(2)
 
     ctx = gpu.Context(0)
