@@ -660,6 +660,18 @@ __device__ __forceinline__ void decode2(double xd, double &x, uint32_t &xs, int 
   xs = lo32(xd) * 16u + 8u * (uint32_t)e;
 }
 
+/* Scheduling hint: interleave the residue arithmetic with the LDS gathers
+   (9 VALU instructions, then 3 DS reads, repeated), so that a wave's first
+   gathers are in flight while it computes the later residues (sieve 177 ->
+   174 ms/K against all residues first, then all reads). */
+template <int NREADS>
+__device__ __forceinline__ void interleave_residues_and_reads() {
+  static_for<0, (NREADS + 2) / 3>([&](auto) {
+    __builtin_amdgcn_sched_group_barrier(0x2, 9, 0);   /* VALU */
+    __builtin_amdgcn_sched_group_barrier(0x100, 3, 0); /* DS read */
+  });
+}
+
 template <int NL, int B, int E>
 __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const char *ldsb, const char *gtb) {
   double x;
@@ -680,6 +692,7 @@ __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const c
     else
       w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
   });
+  if constexpr (E <= NL) interleave_residues_and_reads<E - B>();
   static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
   return m;
 }
@@ -763,6 +776,7 @@ __device__ __forceinline__ void level0_pair(double xd, const char *ldsb, const c
     else
       w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
   });
+  interleave_residues_and_reads<E - B - V>();
   uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
   static_for<0, V>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
