@@ -68,12 +68,12 @@ PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3 T int32 lane-ops/s (nominal)
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
-# workload (profiles/sieve_persistent2_traffic.md): FETCH_SIZE 331,026 KiB and
-# WRITE_SIZE 15,524,955 KiB per launch.  Both are scaled by what the same
+# workload (profiles/sieve_persistent2_traffic.md): FETCH_SIZE 291,960 KiB and
+# WRITE_SIZE 15,961,158 KiB per launch.  Both are scaled by what the same
 # counters read for prp_kernel over 2^26 values, which reads 512 MiB and
 # writes 64 MiB exactly: FETCH_SIZE 262,174 KiB (x2.000), WRITE_SIZE
-# 65,536 KiB (x1.000).  (2 x 331,026 + 15,524,955) KiB / 16 K:
-TRAFFIC_BYTES_PER_K = (2.0 * 331025.9 + 15524955.2) * 1024 / 16
+# 65,536 KiB (x1.000).  (2 x 291,960 + 15,961,158) KiB / 16 K:
+TRAFFIC_BYTES_PER_K = (2.0 * 291960.4 + 15961158.4) * 1024 / 16
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
 # n43 starts (10,840 x 8 B) read once.
