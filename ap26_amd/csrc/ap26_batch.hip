@@ -44,12 +44,21 @@ PersistVariant make_pvariant2() {
   return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, V, G>};
 }
 
-/* 1..3, 9: one shift per item; 0 (default = 28), 10..34: shift pairs.  Level 0
+/* shift pairs with the bank-rotated level-0 group (primes 61..137) */
+template <int NL, int WAVES, class LV>
+PersistVariant make_pvariant2r() {
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N>};
+}
+
+/* 1..3, 9: one shift per item; 10..40, 50..56: shift pairs.  Level 0
    (every position, both shifts of a pair per 16-byte read) runs on the first
-   18..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
+   16..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
    so level 0 pays off further than on one shift (61..131).  35..38: the first
    1..3 level-0 primes from register bit windows; 39..40: the last 2 or 4
-   level-0 primes read through L1 instead of LDS. */
+   level-0 primes read through L1 instead of LDS.  50..56 (0 = 54, the
+   default): level 0 opens with the bank-rotated group of 61..137
+   (level0_rot), whose 64 VGPRs of per-lane constants leave room for 12 waves
+   per CU (16 waves spill in the hot loop, 8 lose the latency hiding). */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -74,7 +83,6 @@ PersistVariant pick_pvariant(int id) {
     case 25: return make_pvariant2<29, 16, Levels<0, 20, 29, 41, 83>>();
     case 26: return make_pvariant2<29, 16, Levels<0, 20, 29, 51, 83>>();
     case 27: return make_pvariant2<29, 16, Levels<0, 24, 29, 45, 83>>();
-    case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     case 29: return make_pvariant2<29, 16, Levels<0, 19, 29, 45, 83>>();
     case 30: return make_pvariant2<29, 16, Levels<0, 20, 29, 55, 83>>();
     case 31: return make_pvariant2<29, 16, Levels<0, 18, 27, 45, 83>>();
@@ -87,7 +95,15 @@ PersistVariant pick_pvariant(int id) {
     case 38: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>, 3>();
     case 39: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 2>();
     case 40: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 4>();
-    default: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
+    case 50: return make_pvariant2r<21, 16, Levels<0, 16, 21, 51, 83>>();
+    case 51: return make_pvariant2r<21, 16, Levels<0, 18, 21, 51, 83>>();
+    case 52: return make_pvariant2r<29, 12, Levels<0, 16, 29, 51, 83>>();
+    case 53: return make_pvariant2r<29, 12, Levels<0, 18, 29, 51, 83>>();
+    case 54: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>>();
+    case 55: return make_pvariant2r<29, 12, Levels<0, 20, 29, 51, 83>>();
+    case 56: return make_pvariant2r<36, 8, Levels<0, 18, 36, 55, 83>>();
+    case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
+    default: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>>(); /* = 54 */
   }
 }
 
@@ -827,7 +843,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {28, 32, 22, 26, 10, 12, 14, 9, 1};
+static const int TUNE_VARIANTS[] = {54, 53, 55, 52, 28, 32, 22, 10, 9};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

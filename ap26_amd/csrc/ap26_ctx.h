@@ -51,6 +51,12 @@ SieveVariant make_q2variant() {
   return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, V>};
 }
 
+/* shift pairs with the bank-rotated level-0 group (primes 61..137) */
+template <int NL, int WAVES, class LV>
+SieveVariant make_q2variantr() {
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_N>};
+}
+
 
 struct BatchState;
 struct ap26_ctx {

@@ -18,11 +18,13 @@ using namespace ap26;
 
 #include "ap26_ctx.h"
 
-/* Sieve variants: 0 = default (31: shift-pair queued sieve, level 0 on 18
-   primes), 1 = simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve
-   tuning candidates (LDS primes, waves per CU, levels, VGPR constants),
-   31..33 = shift-pair queued sieves, 34..36 = shift-pair with the first 1..3
-   level-0 primes from register bit windows instead of LDS. */
+/* Sieve variants: 0 = default (42: shift-pair queued sieve with the
+   bank-rotated level-0 group, level 0 on 18 primes, 12 waves per CU), 1 =
+   simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve tuning
+   candidates (LDS primes, waves per CU, levels, VGPR constants), 31..33 =
+   shift-pair queued sieves without the rotation group, 34..36 = shift-pair
+   with the first 1..3 level-0 primes from register bit windows instead of
+   LDS, 40..44 = rotation-group forms. */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -54,7 +56,12 @@ SieveVariant pick_variant(int id) {
     case 34: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 1>();
     case 35: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 2>();
     case 36: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>, 3>();
-    default: return make_q2variant<29, 16, Levels<0, 18, 29, 51, 83>>();
+    case 40: return make_q2variantr<29, 12, Levels<0, 16, 29, 51, 83>>();
+    case 41: return make_q2variantr<29, 12, Levels<0, 18, 29, 51, 83>>();
+    case 42: return make_q2variantr<30, 12, Levels<0, 18, 30, 51, 83>>();
+    case 43: return make_q2variantr<36, 8, Levels<0, 18, 36, 55, 83>>();
+    case 44: return make_q2variantr<21, 16, Levels<0, 16, 21, 51, 83>>();
+    default: return make_q2variantr<30, 12, Levels<0, 18, 30, 51, 83>>(); /* = 42 */
   }
 }
 

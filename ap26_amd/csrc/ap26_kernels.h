@@ -672,45 +672,177 @@ __device__ __forceinline__ void interleave_residues_and_reads() {
   });
 }
 
-template <int NL, int B, int E>
+/* ---- bank-rotated level 0 (shift-pair sieve) ----------------------------- */
+/* A random 16-byte LDS gather serves the 16 lanes of a pass in about three
+   times the cycles of a conflict-free one: lanes land on the same 4-bank group
+   with different addresses.  The rotated layout removes that for the first
+   ROT_N = 16 sieve primes (61..137): LDS row rho (256 bytes, one per bank
+   sweep) holds in its 16-byte column q the shift-pair entry of prime q for
+   the residue rho - ROT_HALF, so prime q lives in bank group q only.  At
+   step k of level 0, lane j looks up prime (j + k) mod 16 for its own
+   position; the 16 lanes of each pass then read 16 different bank groups,
+   and after 16 steps every lane has ANDed every prime of the group.  The
+   per-lane prime constants (reciprocal, row multiplier, column) live in
+   VGPRs (RotConsts).  With 256-byte rows the i24 multiply-add gives the row
+   offset 256*r exactly modulo 2^32 (256 * 2^24 = 2^32), so no sign
+   extension is needed: |256 r| < 2^31. */
+constexpr int ROT_N = 16;
+constexpr int ROT_HALF = (PT.p[ROT_N - 1] - 1) / 2; /* 68 */
+constexpr int ROT_ROWS = 2 * ROT_HALF + 1;          /* 137 */
+constexpr int ROT_BYTES = 256 * ROT_ROWS;           /* 35,072 */
+static_assert(ROT_N == 16 && PT.p[ROT_N - 1] == 137, "rotation group is 61..137");
+
+/* LDS bytes in front of the packed shift-pair tables, and the first prime
+   index the packed tables hold: R0 = 0 (no rotation group) or ROT_N. */
+template <int R0>
+constexpr int rot_lds_bytes() {
+  return R0 ? ROT_BYTES : 0;
+}
+
+struct RotConsts {
+  double inv[ROT_N]; /* RN(1/p), the constant centred_byteoff uses */
+  int m[ROT_N];      /* -256 p */
+  int col[ROT_N];    /* 16 q: byte offset of prime q's column in a row */
+  __device__ __forceinline__ void init(int lane) {
+    static_for<0, ROT_N>([&](auto KC) {
+      constexpr int k = decltype(KC)::value;
+      const int q = (lane + k) & (ROT_N - 1);
+      int p = PT.p[0];
+      static_for<1, ROT_N>([&](auto QC) {
+        constexpr int Q = decltype(QC)::value;
+        p = q == Q ? PT.p[Q] : p;
+      });
+      inv[k] = 1.0 / (double)p; /* correctly rounded, = the compile-time 1.0/P */
+      m[k] = -256 * p;
+      col[k] = 16 * q;
+    });
+  }
+};
+
+/* Stage one shift pair's tables into LDS: the rotation rows (R0 = ROT_N),
+   the packed entries of primes R0..NL-1 and the 7..23 filters.  tab is the
+   pair's global table, [entry][shift in pair], 16 bytes per entry. */
+template <int R0, int NL>
+__device__ __forceinline__ void stage_pair_tables(char *lds, const uint64_t *__restrict__ tab) {
+  constexpr int RB = rot_lds_bytes<R0>();
+  if constexpr (R0) {
+    const uint4 *t4 = (const uint4 *)tab;
+    uint4 *rows = (uint4 *)lds;
+    for (int i = threadIdx.x; i < ROT_ROWS * ROT_N; i += blockDim.x) {
+      const int q = i & (ROT_N - 1);
+      const int p = PT.p[q];
+      int c = (i >> 4) - ROT_HALF + (p - 1) / 2; /* centred entry of residue row - ROT_HALF */
+      c = (c % p + p) % p;                      /* rows past (p-1)/2 are never read; keep them valid */
+      rows[i] = t4[PT.off[q] + c];
+    }
+  }
+  uint64_t *packed = (uint64_t *)(lds + RB);
+  constexpr int NPK = 2 * (PT.off[NL] - PT.off[R0]);
+  for (int i = threadIdx.x; i < NPK; i += blockDim.x) packed[i] = tab[2 * PT.off[R0] + i];
+  for (int i = threadIdx.x; i < 180; i += blockDim.x) packed[NPK + i] = tab[2 * NOK + i];
+}
+
+/* The primes [B, E) of a queued (position, shift) entry, in chunks of CH
+   primes (residues, loads, ANDs per chunk): with the rotation group's
+   constants resident, the deep levels must not hold all their loads at once. */
+template <int NL, int B, int E, int R0 = 0, int CH = 64>
 __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const char *ldsb, const char *gtb) {
+  static_assert(B >= R0, "rotated primes are level 0 only");
   double x;
   uint32_t xs;
   int e;
   decode2(xd, x, xs, e);
-  int bo[E - B];
-  uint64_t w[E - B];
-  static_for<B, E>([&](auto IC) {
-    constexpr int I = decltype(IC)::value;
-    bo[I - B] = centred_byteoff<PT.p[I], 16>(x, xs);
-  });
-  static_for<B, E>([&](auto IC) {
-    constexpr int I = decltype(IC)::value;
-    constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
-    if constexpr (I < NL)
-      w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
-    else
-      w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
-  });
-  if constexpr (E <= NL) interleave_residues_and_reads<E - B>();
-  static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
-  return m;
+  if constexpr (E - B > CH) {
+    m = apply_primes2<NL, B, B + CH, R0, CH>(m, xd, ldsb, gtb);
+    return apply_primes2<NL, B + CH, E, R0, CH>(m, xd, ldsb, gtb);
+  } else {
+    int bo[E - B];
+    uint64_t w[E - B];
+    static_for<B, E>([&](auto IC) {
+      constexpr int I = decltype(IC)::value;
+      bo[I - B] = centred_byteoff<PT.p[I], 16>(x, xs);
+    });
+    static_for<B, E>([&](auto IC) {
+      constexpr int I = decltype(IC)::value;
+      if constexpr (I < NL) {
+        constexpr int C0 = rot_lds_bytes<R0>() + 16 * (PT.off[I] - PT.off[R0] + (PT.p[I] - 1) / 2);
+        w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
+      } else {
+        constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
+        w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
+      }
+    });
+    if constexpr (E <= NL) interleave_residues_and_reads<E - B>();
+    static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
+    return m;
+  }
 }
 
-template <int NL>
+template <int NL, int R0 = 0>
 __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const char *ldsb) {
   double x;
   uint32_t xs;
   int e;
   decode2(xd, x, xs, e);
-  constexpr int LMAIN = PT.off[NL];
+  constexpr int LMAIN = PT.off[NL] - PT.off[R0];
   static_for<NSP, NPT>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     const int bo = centred_byteoff<PT.p[I], 16>(x, xs);
-    constexpr int C0 = 16 * (LMAIN + PT.off[I] - NOK + (PT.p[I] - 1) / 2);
+    constexpr int C0 = rot_lds_bytes<R0>() + 16 * (LMAIN + PT.off[I] - NOK + (PT.p[I] - 1) / 2);
     m &= *(const uint64_t *)(ldsb + C0 + bo);
   });
   return m;
+}
+
+/* Level 0 of a fresh position with the rotation group: both masks of the
+   pair over primes 0..E-1, the first ROT_N through the rotated rows, the rest
+   (ROT_N..E-1) from the packed LDS tables. */
+template <int E>
+__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConsts &rc, uint64_t &m0,
+                                           uint64_t &m1) {
+  static_assert(E >= ROT_N, "level 0 holds the rotation group");
+  const double x = xd - TWO52;
+  const uint32_t xs256 = lo32(xd) * 256u;
+  const uint32_t xs16 = lo32(xd) * 16u;
+  uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
+  /* two halves of the group, so that at most half its gathers are in flight */
+  static_for<0, 2>([&](auto HC) {
+    constexpr int H = decltype(HC)::value;
+    uint4 w[ROT_N / 2];
+    static_for<0, ROT_N / 2>([&](auto KC) {
+      constexpr int k = H * (ROT_N / 2) + decltype(KC)::value;
+      const double qd = __fma_rn(x, rc.inv[k], SHIFTER);
+      int r; /* 256 * (centred residue of prime (lane + k) mod 16) */
+      asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"((int)lo32(qd)), "v"(rc.m[k]), "v"(xs256));
+      w[decltype(KC)::value] = *(const uint4 *)(ldsb + 256 * ROT_HALF + r + rc.col[k]);
+    });
+    interleave_residues_and_reads<ROT_N / 2>();
+    static_for<0, ROT_N / 2>([&](auto KC) {
+      constexpr int k = decltype(KC)::value;
+      a0 &= w[k].x;
+      a1 &= w[k].y;
+      a2 &= w[k].z;
+      a3 &= w[k].w;
+    });
+  });
+  if constexpr (E > ROT_N) {
+    uint4 w[E - ROT_N];
+    static_for<ROT_N, E>([&](auto IC) {
+      constexpr int I = decltype(IC)::value;
+      const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
+      constexpr int C0 = ROT_BYTES + 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
+      w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
+    });
+    static_for<0, E - ROT_N>([&](auto IC) {
+      constexpr int k = decltype(IC)::value;
+      a0 &= w[k].x;
+      a1 &= w[k].y;
+      a2 &= w[k].z;
+      a3 &= w[k].w;
+    });
+  }
+  m0 = ((uint64_t)a1 << 32) | a0;
+  m1 = ((uint64_t)a3 << 32) | a2;
 }
 
 constexpr int pow_mod_const(int a, int e, int p) {
@@ -859,11 +991,20 @@ __device__ __forceinline__ void close_survivors2(const SurvCursor &cur, int64_t 
    equal priorities for the two phases 180, level 0 alone raised 181. */
 constexpr int PRIO_LEVEL0 = 2, PRIO_LEVEL1 = 3, PRIO_DEEP = 1;
 
-template <int NL, class LV>
+/* Queue capacity per wave: queue 0 holds Q0 entries, the deeper ones 128.
+   With Q0 = 192 both pushes of a position (one per shift of the pair) go in
+   before one drain: a single call site of the drain chain per position
+   (its code and the registers it needs appear once). */
+template <int Q0, class LV>
+constexpr int qsieve2_wave_entries() {
+  return Q0 + 128 * (LV::N - 2);
+}
+
+template <int NL, class LV, int R0 = 0, int Q0 = 128>
 struct QSieve2 {
-  static constexpr int QCAP = 128;
+  static_assert(Q0 == 128 || Q0 == 192, "queue 0 holds one or two pushes past a full wave");
   static constexpr int NQ = LV::N - 1;
-  QEntry (*q)[QCAP];
+  QEntry *qb; /* this wave's queues: queue 0 (Q0 entries), then 128 per level */
   int qn[NQ];
   bool partial; /* a queue step ran with lanes switched off (must never happen) */
   const char *ldsb, *gtb;
@@ -880,9 +1021,11 @@ struct QSieve2 {
     partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
   }
 
+  __device__ __forceinline__ QEntry *q(int qi) const { return qb + (qi == 0 ? 0 : Q0 + 128 * (qi - 1)); }
+
   __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
-    if (pred) q[qi][qn[qi] + lanes_below(bal)] = QEntry{xd, m};
+    if (pred) q(qi)[qn[qi] + lanes_below(bal)] = QEntry{xd, m};
     qn[qi] += __popcll(bal);
     __builtin_amdgcn_wave_barrier();
   }
@@ -890,9 +1033,9 @@ struct QSieve2 {
   template <int L>
   __device__ __forceinline__ void run_level(const QEntry &e, bool pred) {
     uint64_t m = e.m;
-    if (pred) m = apply_primes2<NL, LV::B[L + 1], LV::B[L + 2]>(m, e.xd, ldsb, gtb);
+    if (pred) m = apply_primes2<NL, LV::B[L + 1], LV::B[L + 2], R0, (R0 ? 8 : 64)>(m, e.xd, ldsb, gtb);
     if constexpr (L + 2 == LV::N) {
-      if (pred && m) m = apply_filters2<NL>(m, e.xd, ldsb);
+      if (pred && m) m = apply_filters2<NL, R0>(m, e.xd, ldsb);
       emit_survivors2(pred ? m : 0, e.xd, *a, lane, *cur);
     } else {
       push(L + 1, e.xd, m, pred && m != 0);
@@ -905,7 +1048,7 @@ struct QSieve2 {
     while (qn[L] >= 64) {
       check_full_wave();
       qn[L] -= 64;
-      const QEntry e = q[L][qn[L] + lane];
+      const QEntry e = q(L)[qn[L] + lane];
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_s_setprio(L == 0 ? PRIO_LEVEL1 : PRIO_DEEP);
       run_level<L>(e, true);
@@ -922,7 +1065,7 @@ struct QSieve2 {
         check_full_wave();
         const bool pred = lane < n;
         QEntry e{TWO52, 0};
-        if (pred) e = q[L][lane];
+        if (pred) e = q(L)[lane];
         __builtin_amdgcn_wave_barrier();
         qn[L] = 0;
         run_level<L>(e, pred);
@@ -934,9 +1077,9 @@ struct QSieve2 {
   }
 };
 
-template <int NL, class LV, int V = 0, int G = 0>
-__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveArgs &a, int64_t tid,
-                                               int64_t stride) {
+template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128>
+__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0> &qs, const SieveArgs &a, int64_t tid,
+                                               int64_t stride, const RotConsts *rc = nullptr) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
@@ -950,12 +1093,15 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
       qs.check_full_wave();
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(PRIO_LEVEL0);
-      level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
+      if constexpr (R0)
+        level0_rot<LV::B[1]>(xd, qs.ldsb, *rc, m0, m1);
+      else
+        level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
       __builtin_amdgcn_s_setprio(0);
       m0 = has ? m0 : 0;
       m1 = has ? m1 : 0;
       qs.push(0, xd, m0, m0 != 0);
-      qs.template drain<0>();
+      if constexpr (Q0 == 128) qs.template drain<0>();
       qs.push(0, xd + TWO48, m1, m1 != 0);
       qs.template drain<0>();
       xd += S59d;
@@ -965,27 +1111,30 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV> &qs, const SieveA
   qs.template flush<0>();
 }
 
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128>
 struct QSieve2Lds {
-  uint64_t tab[2 * (PT.off[NL] + 90)];
-  QEntry queues[WAVES][LV::N - 1][128];
+  /* rotation rows (R0 = ROT_N), packed pair entries of primes R0..NL-1, filters */
+  alignas(16) char tab[rot_lds_bytes<R0>() + 16 * (PT.off[NL] - PT.off[R0] + 90)];
+  QEntry queues[WAVES][qsieve2_wave_entries<Q0, LV>()];
 };
 
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128>
 constexpr int qsieve2_lds_bytes() {
-  return 2 * (PT.off[NL] + 90) * 8 + WAVES * (LV::N - 1) * 128 * (int)sizeof(QEntry);
+  return (int)sizeof(QSieve2Lds<NL, WAVES, LV, R0, Q0>);
 }
 
 /* Items (job, shift pair, n43 chunk); tables [entry][shift in pair].  V: the
-   first V level-0 primes from registers (vwin_pair) instead of LDS. */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0>
+   first V level-0 primes from registers (vwin_pair) instead of LDS.  R0 =
+   ROT_N: level 0 starts with the bank-rotated group (level0_rot). */
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_persistent2_kernel(PersistArgs p) {
-  static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  constexpr int Q0 = R0 ? 192 : 128;
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0>() + 64 + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
-  constexpr int LMAIN2 = 2 * PT.off[NL];
+  static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
   constexpr int NPASS = AP26_NSHIFT / 2;
-  __shared__ QSieve2Lds<NL, WAVES, LV> L;
+  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0> L;
   __shared__ unsigned int s_item;
   const RoundState rs = *p.rs;
   const int njobs = min(rs.per_round, rs.njobs - rs.base);
@@ -994,6 +1143,8 @@ sieve_persistent2_kernel(PersistArgs p) {
   __shared__ SurvCursor s_cur[WAVES]; /* per wave, in LDS to spare registers */
   SurvCursor &sc = s_cur[threadIdx.x >> 6];
   sc = SurvCursor{0, 0};
+  RotConsts rc;
+  if constexpr (R0) rc.init(threadIdx.x & 63);
   while (true) {
     __syncthreads();
     if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
@@ -1012,8 +1163,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     const uint64_t *tab = p.tab + ((size_t)jl * NPASS + t) * NTAB * 2;
     const int key = jl * NPASS + t;
     if (key != cur) {
-      for (int i = threadIdx.x; i < LMAIN2; i += blockDim.x) L.tab[i] = tab[i];
-      for (int i = threadIdx.x; i < 180; i += blockDim.x) L.tab[LMAIN2 + i] = tab[2 * NOK + i];
+      stage_pair_tables<R0, NL>(L.tab, tab);
       __syncthreads();
       cur = key;
     }
@@ -1034,46 +1184,48 @@ sieve_persistent2_kernel(PersistArgs p) {
     a.surv_job = p.surv_job;
     a.err = p.err;
     a.job = jl;
-    QSieve2<NL, LV> qs;
+    QSieve2<NL, LV, R0, Q0> qs;
     qs.lane = threadIdx.x & 63;
-    qs.q = L.queues[threadIdx.x >> 6];
-    for (int i = 0; i < QSieve2<NL, LV>::NQ; i++) qs.qn[i] = 0;
+    qs.qb = L.queues[threadIdx.x >> 6];
+    for (int i = 0; i < QSieve2<NL, LV, R0, Q0>::NQ; i++) qs.qn[i] = 0;
     qs.partial = false;
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V, G>(qs, a, threadIdx.x, blockDim.x);
+    qsieve2_blocks<NL, LV, V, G, R0, Q0>(qs, a, threadIdx.x, blockDim.x, &rc);
   }
 }
 
 /* One launch per (K, shift pair): the plain path with the shift-pair sieve
    (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
    both shifts), survivors in per-wave slot chunks as in the batched engine. */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_queue2_kernel(SieveArgs a) {
-  static_assert(qsieve2_lds_bytes<NL, WAVES, LV>() <= 160 * 1024, "LDS budget");
+  constexpr int Q0 = R0 ? 192 : 128;
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0>() + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
-  constexpr int LMAIN2 = 2 * PT.off[NL];
-  __shared__ QSieve2Lds<NL, WAVES, LV> L;
+  static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
+  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0> L;
   __shared__ SurvCursor s_cur[WAVES];
-  for (int i = threadIdx.x; i < LMAIN2; i += blockDim.x) L.tab[i] = a.tab[i];
-  for (int i = threadIdx.x; i < 180; i += blockDim.x) L.tab[LMAIN2 + i] = a.tab[2 * NOK + i];
+  stage_pair_tables<R0, NL>(L.tab, a.tab);
   SurvCursor &sc = s_cur[threadIdx.x >> 6];
   sc = SurvCursor{0, 0};
+  RotConsts rc;
+  if constexpr (R0) rc.init(threadIdx.x & 63);
   __syncthreads();
-  QSieve2<NL, LV> qs;
+  QSieve2<NL, LV, R0, Q0> qs;
   qs.lane = threadIdx.x & 63;
-  qs.q = L.queues[threadIdx.x >> 6];
-  for (int i = 0; i < QSieve2<NL, LV>::NQ; i++) qs.qn[i] = 0;
+  qs.qb = L.queues[threadIdx.x >> 6];
+  for (int i = 0; i < QSieve2<NL, LV, R0, Q0>::NQ; i++) qs.qn[i] = 0;
   qs.partial = false;
   qs.ldsb = (const char *)L.tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV, V, G>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
-                            (int64_t)gridDim.x * blockDim.x);
+  qsieve2_blocks<NL, LV, V, G, R0, Q0>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                       (int64_t)gridDim.x * blockDim.x, &rc);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }
 

@@ -21,6 +21,8 @@ from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from ap26_amd import workloads as W  # noqa: E402
 REF = "/root/reference"
 REF_OUT = os.path.join(ROOT, "oracle", "_ref")
 RUNS = os.path.join(REF_OUT, "runs")
@@ -54,6 +56,15 @@ RUN_LIST = [
     ("mid_300000007", 300000007, 300000007, 0),
     ("s5120_1000003", 1000003, 1000003, 5120),
 ]
+# bench_*: every searched K the benchmark times -- the batched leg's sample of
+# the 256-unit workload for up to 48 steps (ap26_amd.workloads.c4_sample, each
+# K at its unit's SHIFT) and the plain leg's first 24 searched K of the
+# 4096-K unit at SHIFT 640; c3s_*: every 256th searched K of that whole unit.
+_have = {r[1] for r in RUN_LIST if r[1] == r[2]}
+for _u, _ks in zip(*W.c4_sample(48)[::2]):
+    RUN_LIST += [(f"bench_{K}_{_u[2]}", K, K, _u[2]) for K in _ks]
+RUN_LIST += [(f"bench_{K}_640", K, K, 640) for K in W.c3_ks(24) if K not in _have]
+RUN_LIST += [(f"c3s_{K}", K, K, 640) for K in W.c3_stratified(256) if K not in _have]
 
 
 def run_one(name, kmin, kmax, shift):
@@ -96,13 +107,20 @@ def main():
     for f in sorted(os.listdir(os.path.join(REF, "tests"))):
         if f.startswith("TEST-") and f.endswith(".txt"):
             shutil.copy(os.path.join(REF, "tests", f), os.path.join(tdst, f))
+    old = {}
+    if os.path.exists(os.path.join(DST, "index.json")):
+        with open(os.path.join(DST, "index.json")) as f:
+            old = {m["name"]: m for m in json.load(f)}
     if "--run" in sys.argv:
-        todo = [r for r in RUN_LIST if not os.path.exists(os.path.join(RUNS, r[0], "SOL-AP26.txt"))]
-        with ThreadPoolExecutor(max_workers=6) as ex:
+        # runs whose fixture is already committed are not repeated
+        todo = [r for r in RUN_LIST if r[0] not in old and
+                not os.path.exists(os.path.join(RUNS, r[0], "SOL-AP26.txt"))]
+        jobs = int(os.environ.get("AP26_FIXTURE_JOBS", "6"))
+        with ThreadPoolExecutor(max_workers=jobs) as ex:
             list(ex.map(lambda r: run_one(*r), todo))
     index = []
     for r in RUN_LIST:
-        meta = collect(*r)
+        meta = collect(*r) or old.get(r[0])
         if meta:
             index.append(meta)
         else:
