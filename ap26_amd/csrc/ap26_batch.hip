@@ -109,6 +109,23 @@ PersistVariant pick_pvariant(int id) {
 
 }  // namespace
 
+/* Per-round inputs and outputs.  The batched call uses io[0]; the server
+   alternates io[0] and io[1], so that one round runs while the host harvests
+   the one before.  The large buffers (tables, n43 starts, survivors) are
+   shared: rounds run one after another on the context's stream. */
+struct RoundIO {
+  BatchJobDev *d_jobs = nullptr;
+  RoundState *d_rs = nullptr;
+  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) [4] overflow [5] survivors [6] err */
+  DevSol *d_sols = nullptr;
+  unsigned long long *d_hash = nullptr; /* per range: exact sum of first%1000 + k */
+  unsigned long long *d_big = nullptr;  /* per range: walked terms >= 2^62 */
+  /* one round = graph A (n43 starts, tables, persistent sieve) then graph B
+     (walk, advance); events recorded between them time the sieve side */
+  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph_b = nullptr;
+};
+
 struct BatchState {
   int per_round = 0;
   int chunk = 0, nchunk = 0;
@@ -116,55 +133,59 @@ struct BatchState {
   PersistVariant pv{};
   int64_t surv_cap = 0;      /* survivor slots per round */
   int64_t surv_slack = 0;    /* slots the sieve's waves may reserve and leave unused */
-  int max_jobs = 0;          /* capacity of d_jobs */
-  int max_ranges = 0;
-  BatchJobDev *d_jobs = nullptr;
-  RoundState *d_rs = nullptr;
+  int max_jobs = 0;          /* capacity of each io's d_jobs */
+  int max_ranges = 0;        /* capacity of each io's d_hash and d_big */
+  int nio = 1;
+  RoundIO io[2];
   uint32_t *d_triples = nullptr;
   int64_t *d_n43 = nullptr;
   uint64_t *d_tab = nullptr;
   int64_t *d_surv = nullptr;
   int32_t *d_surv_job = nullptr;
-  unsigned long long *d_cnt = nullptr; /* [0] nsurv [1] tests [2] nsol(u32) [3] work(u32) [4] overflow [5] survivors [6] err */
-  DevSol *d_sols = nullptr;
-  unsigned long long *d_hash = nullptr;
-  /* one round = graph A (n43 starts, tables, persistent sieve) then graph B
-     (walk, advance); events recorded between them time the sieve side */
-  hipGraphExec_t graph = nullptr;   /* A */
-  hipGraphExec_t graph_b = nullptr; /* B */
-  std::vector<hipEvent_t> ev;       /* 2 per timed round */
+  std::vector<hipEvent_t> ev; /* 2 per timed round */
 };
+
+static void drop_graphs(BatchState *b) {
+  for (RoundIO &o : b->io) {
+    if (o.graph) (void)hipGraphExecDestroy(o.graph);
+    if (o.graph_b) (void)hipGraphExecDestroy(o.graph_b);
+    o.graph = o.graph_b = nullptr;
+  }
+}
 
 static void batch_state_free(BatchState *b) {
   if (!b) return;
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
+  drop_graphs(b);
   for (auto &e : b->ev) (void)hipEventDestroy(e);
-  (void)hipFree(b->d_jobs);
-  (void)hipFree(b->d_rs);
+  for (RoundIO &o : b->io) {
+    (void)hipFree(o.d_jobs);
+    (void)hipFree(o.d_rs);
+    (void)hipFree(o.d_cnt);
+    (void)hipFree(o.d_sols);
+    (void)hipFree(o.d_hash);
+    (void)hipFree(o.d_big);
+  }
   (void)hipFree(b->d_triples);
   (void)hipFree(b->d_n43);
   (void)hipFree(b->d_tab);
   (void)hipFree(b->d_surv);
   (void)hipFree(b->d_surv_job);
-  (void)hipFree(b->d_cnt);
-  (void)hipFree(b->d_sols);
-  (void)hipFree(b->d_hash);
   delete b;
 }
 
-/* per_round = K held by one round (device memory ~0.75 GB per K) */
+/* per_round = K held by one round (device memory ~0.61 GB per K) */
 void batch_free(ap26_ctx *ctx) {
   batch_state_free(ctx->batch);
   ctx->batch = nullptr;
 }
 
-static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
+static int batch_state_new(ap26_ctx *ctx, int per_round, int nio, BatchState **out) {
   *out = nullptr;
   BatchState *b = new (std::nothrow) BatchState();
   if (!b) return AP26_E_NOMEM;
   int rc = AP26_OK;
   b->per_round = per_round;
+  b->nio = nio;
   /* environment overrides > autotuned choice > defaults */
   const char *e = getenv("AP26_BATCH_CHUNK");
   b->chunk = e ? atoi(e) : (ctx->tune_chunk > 0 ? ctx->tune_chunk : 85);
@@ -186,14 +207,16 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, BatchState **out) {
       }
     }
   if (trip.size() * 8 != AP26_NN43) rc = AP26_E_INTERNAL;
-  if (!rc && (hipMalloc(&b->d_rs, sizeof(RoundState)) != hipSuccess ||
-              hipMalloc(&b->d_triples, trip.size() * 4) != hipSuccess ||
+  for (int i = 0; i < nio && !rc; i++)
+    if (hipMalloc(&b->io[i].d_rs, sizeof(RoundState)) != hipSuccess ||
+        hipMalloc(&b->io[i].d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
+        hipMalloc(&b->io[i].d_sols, sizeof(DevSol) * BATCH_SOL_CAP) != hipSuccess)
+      rc = AP26_E_NOMEM;
+  if (!rc && (hipMalloc(&b->d_triples, trip.size() * 4) != hipSuccess ||
               hipMalloc(&b->d_n43, sizeof(int64_t) * AP26_NN43 * b->per_round) != hipSuccess ||
               hipMalloc(&b->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT * b->per_round) != hipSuccess ||
               hipMalloc(&b->d_surv, sizeof(int64_t) * b->surv_cap) != hipSuccess ||
-              hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess ||
-              hipMalloc(&b->d_cnt, sizeof(unsigned long long) * 8) != hipSuccess ||
-              hipMalloc(&b->d_sols, sizeof(DevSol) * BATCH_SOL_CAP) != hipSuccess))
+              hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess))
     rc = AP26_E_NOMEM;
   if (!rc && hipMemcpy(b->d_triples, trip.data(), trip.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     rc = AP26_E_DEVICE;
@@ -210,35 +233,41 @@ static int batch_init(ap26_ctx *ctx) {
   const char *e = getenv("AP26_BATCH_ROUND");
   int per_round = e ? atoi(e) : 16;
   if (per_round < 1 || per_round > 64) per_round = 16;
-  return batch_state_new(ctx, per_round, &ctx->batch);
+  return batch_state_new(ctx, per_round, 1, &ctx->batch);
 }
 
 static int ensure_capacity(BatchState *b, int njobs, int nranges) {
   if (njobs > b->max_jobs) {
-    (void)hipFree(b->d_jobs);
-    b->d_jobs = nullptr;
-    int cap = njobs < 1024 ? 1024 : njobs;
-    if (hipMalloc(&b->d_jobs, sizeof(BatchJobDev) * cap) != hipSuccess) return AP26_E_NOMEM;
+    drop_graphs(b);
+    b->max_jobs = 0;
+    const int cap = njobs < 1024 ? 1024 : njobs;
+    for (int i = 0; i < b->nio; i++) {
+      (void)hipFree(b->io[i].d_jobs);
+      b->io[i].d_jobs = nullptr;
+      if (hipMalloc(&b->io[i].d_jobs, sizeof(BatchJobDev) * cap) != hipSuccess) return AP26_E_NOMEM;
+    }
     b->max_jobs = cap;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
-    b->graph = b->graph_b = nullptr;
   }
   if (nranges > b->max_ranges) {
-    (void)hipFree(b->d_hash);
-    b->d_hash = nullptr;
-    int cap = nranges < 1024 ? 1024 : nranges;
-    if (hipMalloc(&b->d_hash, sizeof(unsigned long long) * cap) != hipSuccess) return AP26_E_NOMEM;
+    drop_graphs(b);
+    b->max_ranges = 0;
+    const int cap = nranges < 1024 ? 1024 : nranges;
+    for (int i = 0; i < b->nio; i++) {
+      RoundIO &o = b->io[i];
+      (void)hipFree(o.d_hash);
+      (void)hipFree(o.d_big);
+      o.d_hash = o.d_big = nullptr;
+      if (hipMalloc(&o.d_hash, sizeof(unsigned long long) * cap) != hipSuccess ||
+          hipMalloc(&o.d_big, sizeof(unsigned long long) * cap) != hipSuccess)
+        return AP26_E_NOMEM;
+    }
     b->max_ranges = cap;
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
-    b->graph = b->graph_b = nullptr;
   }
   return AP26_OK;
 }
 
 /* Grow the per-round survivor buffers to `need` slots; the graphs hold the
-   old pointers and are captured again. */
+   old pointers and are captured again.  Waits for the rounds in flight. */
 static int ensure_surv_cap(ap26_ctx *ctx, BatchState *b, int64_t need) {
   if (need <= b->surv_cap) return AP26_OK;
   HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -247,9 +276,7 @@ static int ensure_surv_cap(ap26_ctx *ctx, BatchState *b, int64_t need) {
   b->d_surv = nullptr;
   b->d_surv_job = nullptr;
   b->surv_cap = 0;
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  if (b->graph_b) (void)hipGraphExecDestroy(b->graph_b);
-  b->graph = b->graph_b = nullptr;
+  drop_graphs(b);
   if (hipMalloc(&b->d_surv, sizeof(int64_t) * need) != hipSuccess ||
       hipMalloc(&b->d_surv_job, sizeof(int32_t) * need) != hipSuccess) {
     (void)hipFree(b->d_surv);
@@ -264,54 +291,55 @@ static int ensure_surv_cap(ap26_ctx *ctx, BatchState *b, int64_t need) {
 
 /* enqueue one round on ctx->stream (captured into the graphs): part 0 =
    n43 starts, tables, sieve; part 1 = walk, advance */
-static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part) {
+static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part, const RoundIO &o) {
   const int R = b->per_round;
-  if (part == 1) goto walk;
-  hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, b->d_jobs,
-                     b->d_rs, b->d_triples, b->d_n43);
-  hipLaunchKernelGGL(batch_tables_kernel, dim3((541 + 63) / 64, NPT, R * AP26_NSHIFT), dim3(64), 0,
-                     ctx->stream, b->d_jobs, b->d_rs, b->d_tab, b->pv.S);
-  PersistArgs p;
-  p.jobs = b->d_jobs;
-  p.rs = b->d_rs;
-  p.n43 = b->d_n43;
-  p.tab = b->d_tab;
-  p.surv = b->d_surv;
-  p.surv_job = b->d_surv_job;
-  p.nsurv = b->d_cnt + 0;
-  p.cap = b->surv_cap;
-  p.work = (unsigned int *)(b->d_cnt + 3);
-  p.err = (unsigned int *)(b->d_cnt + 6);
-  p.chunk = b->chunk;
-  p.nchunk = b->nchunk;
-  hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
-  return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
-walk:
+  if (part == 0) {
+    hipLaunchKernelGGL(n43_kernel, dim3((R * AP26_NN43 + 255) / 256), dim3(256), 0, ctx->stream, o.d_jobs, o.d_rs,
+                       b->d_triples, b->d_n43);
+    hipLaunchKernelGGL(batch_tables_kernel, dim3((541 + 63) / 64, NPT, R * AP26_NSHIFT), dim3(64), 0,
+                       ctx->stream, o.d_jobs, o.d_rs, b->d_tab, b->pv.S);
+    PersistArgs p;
+    p.jobs = o.d_jobs;
+    p.rs = o.d_rs;
+    p.n43 = b->d_n43;
+    p.tab = b->d_tab;
+    p.surv = b->d_surv;
+    p.surv_job = b->d_surv_job;
+    p.nsurv = o.d_cnt + 0;
+    p.cap = b->surv_cap;
+    p.work = (unsigned int *)(o.d_cnt + 3);
+    p.err = (unsigned int *)(o.d_cnt + 6);
+    p.chunk = b->chunk;
+    p.nchunk = b->nchunk;
+    hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
+    return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
+  }
   BatchWalkArgs w;
-  w.jobs = b->d_jobs;
-  w.rs = b->d_rs;
+  w.jobs = o.d_jobs;
+  w.rs = o.d_rs;
   w.surv = b->d_surv;
   w.surv_job = b->d_surv_job;
-  w.nsurv = b->d_cnt + 0;
+  w.nsurv = o.d_cnt + 0;
   w.cap = b->surv_cap;
-  w.sols = b->d_sols;
-  w.nsol = (unsigned int *)(b->d_cnt + 2);
+  w.sols = o.d_sols;
+  w.nsol = (unsigned int *)(o.d_cnt + 2);
   w.sol_cap = BATCH_SOL_CAP;
-  w.ntests = b->d_cnt + 1;
-  w.hash_tot = b->d_hash;
-  w.overflow = b->d_cnt + 4;
-  w.nlive = b->d_cnt + 5;
-  w.err = (unsigned int *)(b->d_cnt + 6);
+  w.ntests = o.d_cnt + 1;
+  w.hash_tot = o.d_hash;
+  w.overflow = o.d_cnt + 4;
+  w.nlive = o.d_cnt + 5;
+  w.err = (unsigned int *)(o.d_cnt + 6);
+  w.big = o.d_big;
   hipLaunchKernelGGL(batch_walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
-  hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_rs, b->d_cnt + 0,
-                     (unsigned int *)(b->d_cnt + 3));
+  hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, o.d_rs, o.d_cnt + 0,
+                     (unsigned int *)(o.d_cnt + 3));
   return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
 }
 
-static int capture(ap26_ctx *ctx, BatchState *b, int part, hipGraphExec_t *out) {
+static int capture(ap26_ctx *ctx, BatchState *b, int part, const RoundIO &o, hipGraphExec_t *out) {
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_round(ctx, b, part);
+  int rc = enqueue_round(ctx, b, part, o);
   hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
   if (rc || ce != hipSuccess) return AP26_E_DEVICE;
   hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
@@ -319,19 +347,20 @@ static int capture(ap26_ctx *ctx, BatchState *b, int part, hipGraphExec_t *out) 
   return ie == hipSuccess ? AP26_OK : AP26_E_DEVICE;
 }
 
-static int ensure_graph(ap26_ctx *ctx, BatchState *b) {
-  if (b->graph && b->graph_b) return AP26_OK;
-  int rc = capture(ctx, b, 0, &b->graph);
-  if (!rc) rc = capture(ctx, b, 1, &b->graph_b);
+static int ensure_graph(ap26_ctx *ctx, BatchState *b, int i) {
+  RoundIO &o = b->io[i];
+  if (o.graph && o.graph_b) return AP26_OK;
+  int rc = o.graph ? AP26_OK : capture(ctx, b, 0, o, &o.graph);
+  if (!rc && !o.graph_b) rc = capture(ctx, b, 1, o, &o.graph_b);
   return rc;
 }
 
-/* launch one round; with ev0/ev1 recorded around the sieve graph */
-static int launch_round(ap26_ctx *ctx, BatchState *b, hipEvent_t ev0, hipEvent_t ev1) {
+/* launch one round of io i; with ev0/ev1 recorded around the sieve graph */
+static int launch_round(ap26_ctx *ctx, BatchState *b, int i, hipEvent_t ev0, hipEvent_t ev1) {
   if (ev0) HIPCHK(hipEventRecord(ev0, ctx->stream));
-  HIPCHK(hipGraphLaunch(b->graph, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->io[i].graph, ctx->stream));
   if (ev1) HIPCHK(hipEventRecord(ev1, ctx->stream));
-  HIPCHK(hipGraphLaunch(b->graph_b, ctx->stream));
+  HIPCHK(hipGraphLaunch(b->io[i].graph_b, ctx->stream));
   return AP26_OK;
 }
 
@@ -356,13 +385,15 @@ static BatchJobDev make_job(int32_t K, int32_t shift, int32_t it0, int32_t range
   return j;
 }
 
-static bool unit_ok(const ap26_range &u) {
+/* walk: the unit is searched (walked and reported), not only sieved; STEP,
+   and with it every bound, is largest at kmax */
+static bool unit_ok(const ap26_range &u, bool walk = true) {
   return !(u.kmin < 1 || u.kmax > 600000000 || u.kmin > u.kmax + 1 || !host_shift_ok(u.kmax, u.shift) ||
-           u.iter_start < 0 || u.iter_start > AP26_NSHIFT);
+           (walk && !host_walk_ok(u.kmax, u.shift)) || u.iter_start < 0 || u.iter_start > AP26_NSHIFT);
 }
 
 __global__ void batch_reset_kernel(RoundState *rs, int njobs, int per_round, unsigned long long *cnt,
-                                   unsigned long long *hash, int nranges) {
+                                   unsigned long long *hash, unsigned long long *big, int nranges) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     rs->base = 0;
@@ -371,7 +402,10 @@ __global__ void batch_reset_kernel(RoundState *rs, int njobs, int per_round, uns
     rs->pad = 0;
   }
   if (i < 8) cnt[i] = 0;
-  if (i < nranges) hash[i] = 0;
+  if (i < nranges) {
+    hash[i] = 0;
+    big[i] = 0;
+  }
 }
 
 extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr, ap26_sol *out, int32_t cap,
@@ -421,7 +455,7 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
   rc = ensure_capacity(b, (int)std::min<size_t>(jobs.size(), (size_t)seg_k), nr);
   if (rc) return rc;
   std::vector<DevSol> hs;
-  std::vector<unsigned long long> tot_all(nr, 0), tot(nr);
+  std::vector<unsigned long long> tot_all(nr, 0), tot(nr), big(nr);
   unsigned long long cnt[8];
   int timed_total = 0;
   for (size_t seg0 = 0; seg0 < jobs.size();) {
@@ -446,30 +480,42 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     for (int attempt = 0;; attempt++) {
       rc = ensure_surv_cap(ctx, b, need);
       if (rc) return rc;
-      HIPCHK(hipMemcpyAsync(b->d_jobs, sj, sizeof(BatchJobDev) * segn, hipMemcpyHostToDevice, ctx->stream));
+      HIPCHK(hipMemcpyAsync(b->io[0].d_jobs, sj, sizeof(BatchJobDev) * segn, hipMemcpyHostToDevice, ctx->stream));
       const int nres = nr > 8 ? nr : 8;
-      hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->d_rs, segn,
-                         b->per_round, b->d_cnt, b->d_hash, nr);
+      hipLaunchKernelGGL(batch_reset_kernel, dim3((nres + 255) / 256), dim3(256), 0, ctx->stream, b->io[0].d_rs, segn,
+                         b->per_round, b->io[0].d_cnt, b->io[0].d_hash, b->io[0].d_big, nr);
       HIPCHK(hipGetLastError());
-      rc = ensure_graph(ctx, b);
+      rc = ensure_graph(ctx, b, 0);
       if (rc) return rc;
       HIPCHK(hipEventRecord(ctx->ev[6], ctx->stream));
       for (int i = 0; i < rounds; i++) {
-        rc = i < timed ? launch_round(ctx, b, b->ev[2 * i], b->ev[2 * i + 1])
-                       : launch_round(ctx, b, nullptr, nullptr);
+        rc = i < timed ? launch_round(ctx, b, 0, b->ev[2 * i], b->ev[2 * i + 1])
+                       : launch_round(ctx, b, 0, nullptr, nullptr);
         if (rc) return rc;
       }
-      hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, b->d_sols,
-                         (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
-      hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, b->d_sols,
-                         (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
+      hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, b->io[0].d_sols,
+                         (const unsigned int *)(b->io[0].d_cnt + 2), BATCH_SOL_CAP);
+      hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, b->io[0].d_sols,
+                         (const unsigned int *)(b->io[0].d_cnt + 2), BATCH_SOL_CAP);
       HIPCHK(hipGetLastError());
       HIPCHK(hipEventRecord(ctx->ev[7], ctx->stream));
-      HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHK(hipMemcpyAsync(tot.data(), b->d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
+      HIPCHK(hipMemcpyAsync(cnt, b->io[0].d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHK(hipMemcpyAsync(tot.data(), b->io[0].d_hash, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
+                            ctx->stream));
+      HIPCHK(hipMemcpyAsync(big.data(), b->io[0].d_big, sizeof(unsigned long long) * nr, hipMemcpyDeviceToHost,
                             ctx->stream));
       HIPCHK(hipStreamSynchronize(ctx->stream));
       if (cnt[6]) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
+      {
+        /* a walk reached 2^62, where the reference's PrimeQ exits: no reports */
+        int64_t nbig = 0;
+        for (int i = 0; i < nr; i++) nbig += (int64_t)big[i];
+        if (nbig) {
+          st.walk_past_limit = nbig;
+          if (stats) *stats = st;
+          return AP26_E_RANGE;
+        }
+      }
       /* a round kept more survivors than its budget: grow the buffers to the
          largest count seen and run the segment again */
       if (cnt[4] == 0) break;
@@ -484,7 +530,7 @@ extern "C" int ap26_search_batch(ap26_ctx *ctx, const ap26_range *r, int32_t nr,
     if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL; /* one K, > 2^20 reports */
     const size_t h0 = hs.size();
     hs.resize(h0 + nsol);
-    if (nsol) HIPCHK(hipMemcpy(hs.data() + h0, b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+    if (nsol) HIPCHK(hipMemcpy(hs.data() + h0, b->io[0].d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
     for (int i = 0; i < nr; i++) tot_all[i] += tot[i];
     st.prp_tests += (int64_t)cnt[1];
     st.survivors += (int64_t)cnt[5];
@@ -528,7 +574,7 @@ extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32
   if (!ctx || !n_out || cap < 0 || (cap > 0 && !out)) return AP26_E_RANGE;
   *n_out = 0;
   const ap26_range u{K, K, SHIFT, iter_start};
-  if (!unit_ok(u) || K < 1 || !host_will_search(K)) return AP26_E_RANGE;
+  if (!unit_ok(u, false) || K < 1 || !host_will_search(K)) return AP26_E_RANGE;
   HIPCHK(hipSetDevice(ctx->device));
   int rc = batch_init(ctx);
   if (rc) return rc;
@@ -537,15 +583,15 @@ extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32
   if (!rc) rc = ensure_surv_cap(ctx, b, host_surv_budget(K) + b->surv_slack);
   if (rc) return rc;
   const BatchJobDev job = make_job(K, SHIFT, iter_start, 0);
-  HIPCHK(hipMemcpy(b->d_jobs, &job, sizeof(job), hipMemcpyHostToDevice));
-  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, b->d_rs, 1, b->per_round, b->d_cnt,
-                     b->d_hash, 1);
+  HIPCHK(hipMemcpy(b->io[0].d_jobs, &job, sizeof(job), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(256), 0, ctx->stream, b->io[0].d_rs, 1, b->per_round, b->io[0].d_cnt,
+                     b->io[0].d_hash, b->io[0].d_big, 1);
   HIPCHK(hipGetLastError());
-  rc = ensure_graph(ctx, b);
+  rc = ensure_graph(ctx, b, 0);
   if (rc) return rc;
-  HIPCHK(hipGraphLaunch(b->graph, ctx->stream)); /* n43 starts, tables, sieve: no walk */
+  HIPCHK(hipGraphLaunch(b->io[0].graph, ctx->stream)); /* n43 starts, tables, sieve: no walk */
   unsigned long long cnt[8];
-  HIPCHK(hipMemcpyAsync(cnt, b->d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(cnt, b->io[0].d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(hipStreamSynchronize(ctx->stream));
   const unsigned long long nres = cnt[0];
   if (cnt[6] || nres > (unsigned long long)b->surv_cap) return AP26_E_INTERNAL;
@@ -562,17 +608,50 @@ extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32
 }
 
 /* ---- serving ------------------------------------------------------------ */
+/* ---- serving ------------------------------------------------------------ */
+/* Continuous batching.  Units are admitted by slots and by a device-memory
+   budget (R K per round); each round takes the next K of every active unit
+   in turn.  Two rounds are in flight: pump() keeps a second round queued on
+   the stream behind the running one and only then waits for the older
+   round, so the device never idles while the host harvests a round, admits
+   new units and builds the next.  A round's outputs (counters, reports,
+   per-slot hash sums and 2^62 counts) live in its own RoundIO and are copied
+   to pinned host buffers on the stream; per-unit hash totals are summed on
+   the host from the per-round sums, so a round can be run again (a survivor
+   buffer that overflowed is grown first) without undoing anything on the
+   device. */
+
+constexpr int SERVER_SOL_COPY = 1 << 14; /* reports per round copied back with the round */
 
 struct SUnit {
   ap26_range r{};
   std::vector<int32_t> ks; /* searched K */
   size_t next = 0;         /* next K to schedule */
-  int outstanding = 0;     /* K in the running round */
+  int outstanding = 0;     /* K in rounds in flight */
   int slot = -1;
   double t_submit = 0, t_done = 0;
   std::vector<ap26_sol> sols;
-  int64_t hash = 0;
+  int64_t hash = 0;        /* exact total over the harvested rounds */
+  int status = AP26_OK;    /* AP26_E_RANGE: a walk reached 2^62 */
   bool done = false;
+};
+
+struct SRound {
+  int io = 0;
+  std::vector<int64_t> owner;     /* ticket of each job */
+  std::vector<BatchJobDev> jobs;
+  int64_t slots = 0;              /* survivor slots it needs */
+  bool rerun = false;
+};
+
+struct SPinned {                  /* pinned host side of one RoundIO */
+  BatchJobDev *jobs = nullptr;
+  RoundState *rs = nullptr;
+  unsigned long long *cnt = nullptr;
+  unsigned long long *hash = nullptr;
+  unsigned long long *big = nullptr;
+  DevSol *sols = nullptr;
+  hipEvent_t done = nullptr;
 };
 
 struct ap26_server {
@@ -586,31 +665,38 @@ struct ap26_server {
   int64_t next_ticket = 1;
   int64_t units_done = 0, k_done = 0, rounds = 0;
   size_t rr = 0;                  /* round-robin cursor */
-  BatchJobDev *h_jobs = nullptr;  /* pinned staging */
-  RoundState *h_rs = nullptr;
-  unsigned long long *h_cnt = nullptr;
-  unsigned long long *h_hash = nullptr;
+  SPinned pin[2];
+  std::deque<SRound> inflight;    /* oldest first, at most 2 */
+  int next_io = 0;
 };
 
 static double now_ms() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-__global__ void server_zero_kernel(unsigned long long *cnt) {
-  if (threadIdx.x < 8) cnt[threadIdx.x] = 0;
-}
-
-__global__ void server_zero_slot_kernel(unsigned long long *hash, int slot) {
-  if (threadIdx.x == 0) hash[slot] = 0;
+/* clear one RoundIO's counters and per-slot sums before its round */
+__global__ void server_zero_kernel(unsigned long long *cnt, unsigned long long *hash, unsigned long long *big,
+                                   int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 8) cnt[i] = 0;
+  if (i < n) {
+    hash[i] = 0;
+    big[i] = 0;
+  }
 }
 
 static void server_free(ap26_server *s) {
   if (!s) return;
   batch_state_free(s->b);
-  if (s->h_jobs) (void)hipHostFree(s->h_jobs);
-  if (s->h_rs) (void)hipHostFree(s->h_rs);
-  if (s->h_cnt) (void)hipHostFree(s->h_cnt);
-  if (s->h_hash) (void)hipHostFree(s->h_hash);
+  for (SPinned &p : s->pin) {
+    if (p.jobs) (void)hipHostFree(p.jobs);
+    if (p.rs) (void)hipHostFree(p.rs);
+    if (p.cnt) (void)hipHostFree(p.cnt);
+    if (p.hash) (void)hipHostFree(p.hash);
+    if (p.big) (void)hipHostFree(p.big);
+    if (p.sols) (void)hipHostFree(p.sols);
+    if (p.done) (void)hipEventDestroy(p.done);
+  }
   delete s;
 }
 
@@ -631,20 +717,24 @@ extern "C" int ap26_server_open(ap26_ctx *ctx, int32_t max_units, int64_t mem_bu
   }
   if (R > 64) R = 64;
   s->R = R;
-  int rc = batch_state_new(ctx, R, &s->b);
+  int rc = batch_state_new(ctx, R, 2, &s->b);
   if (!rc) rc = ensure_capacity(s->b, R, max_units);
-  if (!rc && (hipHostMalloc((void **)&s->h_jobs, sizeof(BatchJobDev) * R, hipHostMallocDefault) != hipSuccess ||
-              hipHostMalloc((void **)&s->h_rs, sizeof(RoundState), hipHostMallocDefault) != hipSuccess ||
-              hipHostMalloc((void **)&s->h_cnt, sizeof(unsigned long long) * 8, hipHostMallocDefault) != hipSuccess ||
-              hipHostMalloc((void **)&s->h_hash, sizeof(unsigned long long) * max_units, hipHostMallocDefault) !=
-                  hipSuccess))
-    rc = AP26_E_NOMEM;
-  if (!rc) {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3((max_units + 255) / 256), dim3(256), 0, ctx->stream, s->b->d_rs, 0,
-                       R, s->b->d_cnt, s->b->d_hash, max_units);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) rc = AP26_E_DEVICE;
+  for (int i = 0; i < 2 && !rc; i++) {
+    SPinned &p = s->pin[i];
+    if (hipHostMalloc((void **)&p.jobs, sizeof(BatchJobDev) * R, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&p.rs, sizeof(RoundState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&p.cnt, sizeof(unsigned long long) * 8, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&p.hash, sizeof(unsigned long long) * max_units, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&p.big, sizeof(unsigned long long) * max_units, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&p.sols, sizeof(DevSol) * SERVER_SOL_COPY, hipHostMallocDefault) != hipSuccess)
+      rc = AP26_E_NOMEM;
+    else if (hipEventCreateWithFlags(&p.done, hipEventDisableTiming) != hipSuccess)
+      rc = AP26_E_DEVICE;
   }
-  if (!rc) rc = ensure_graph(ctx, s->b);
+  if (!rc) rc = ensure_graph(ctx, s->b, 0);
+  if (!rc) rc = ensure_graph(ctx, s->b, 1);
   if (rc) {
     server_free(s);
     return rc;
@@ -671,17 +761,15 @@ extern "C" int ap26_server_submit(ap26_server *s, const ap26_range *unit, int64_
   u.r = *unit;
   for (int64_t K = unit->kmin; K <= unit->kmax; K++)
     if (host_will_search((int32_t)K)) u.ks.push_back((int32_t)K);
-  u.slot = s->free_slots.back();
-  s->free_slots.pop_back();
   u.t_submit = now_ms();
   const int64_t t = s->next_ticket++;
-  if (u.ks.empty()) { /* nothing to search: done at once */
+  if (u.ks.empty() || (u.ks.size() == 1 && unit->iter_start >= AP26_NSHIFT)) { /* nothing to search */
     u.done = true;
     u.t_done = u.t_submit;
-    s->free_slots.push_back(u.slot);
-    u.slot = -1;
     s->units_done++;
   } else {
+    u.slot = s->free_slots.back();
+    s->free_slots.pop_back();
     s->active.push_back(t);
   }
   s->units.emplace(t, std::move(u));
@@ -689,108 +777,183 @@ extern "C" int ap26_server_submit(ap26_server *s, const ap26_range *unit, int64_
   return AP26_OK;
 }
 
-extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
-  if (!s) return AP26_E_RANGE;
+/* Enqueue round r on its io: clear the io's outputs, upload the jobs, run
+   the round's graphs and the report verification, copy the outputs to the
+   io's pinned buffers and record the io's event.  Nothing waits. */
+static int server_launch(ap26_server *s, const SRound &r) {
   ap26_ctx *ctx = s->ctx;
   BatchState *b = s->b;
-  if (completed) *completed = 0;
-  if (s->active.empty()) return AP26_OK;
-  HIPCHK(hipSetDevice(ctx->device));
-  /* one K from each active unit in turn until the round is full: R jobs, or
-     the survivor slots (a K divisible by sieve primes takes more of them;
-     one that needs more than the buffers hold alone grows them) */
-  std::vector<int64_t> owner;
-  int n = 0;
-  int64_t slots = b->surv_slack;
+  const RoundIO &o = b->io[r.io];
+  SPinned &p = s->pin[r.io];
+  const int n = (int)r.jobs.size();
+  int rc = ensure_surv_cap(ctx, b, r.slots);
+  if (!rc) rc = ensure_graph(ctx, b, r.io);
+  if (rc) return rc;
+  std::memcpy(p.jobs, r.jobs.data(), sizeof(BatchJobDev) * n);
+  *p.rs = RoundState{0, n, s->R, 0};
+  const int nz = s->max_units > 8 ? s->max_units : 8;
+  hipLaunchKernelGGL(server_zero_kernel, dim3((nz + 255) / 256), dim3(256), 0, ctx->stream, o.d_cnt, o.d_hash, o.d_big,
+                     s->max_units);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(o.d_jobs, p.jobs, sizeof(BatchJobDev) * n, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(hipMemcpyAsync(o.d_rs, p.rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
+  rc = launch_round(ctx, b, r.io, nullptr, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, o.d_sols,
+                     (const unsigned int *)(o.d_cnt + 2), BATCH_SOL_CAP);
+  hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, o.d_sols,
+                     (const unsigned int *)(o.d_cnt + 2), BATCH_SOL_CAP);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(p.cnt, o.d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipMemcpyAsync(p.hash, o.d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(p.big, o.d_big, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPCHK(hipMemcpyAsync(p.sols, o.d_sols, sizeof(DevSol) * SERVER_SOL_COPY, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipEventRecord(p.done, ctx->stream));
+  return AP26_OK;
+}
+
+/* Build the next round (one K from each active unit in turn until R jobs or
+   the survivor slots are taken; a K divisible by sieve primes takes more
+   slots, and one that needs more than the buffers hold alone grows them) and
+   launch it.  Returns 1 if a round was launched, 0 if no unit has K left,
+   or an error, in which case the units' scheduling is left as it was. */
+static int server_enqueue(ap26_server *s) {
+  BatchState *b = s->b;
+  SRound r;
+  r.io = s->next_io;
+  r.slots = b->surv_slack;
+  std::map<int64_t, size_t> taken; /* K taken per unit, in order from its next */
   bool progress = true, full = false;
-  while (n < s->R && progress && !full) {
+  while ((int)r.jobs.size() < s->R && progress && !full) {
     progress = false;
-    for (size_t c = 0; c < s->active.size() && n < s->R; c++) {
-      const size_t idx = (s->rr + c) % s->active.size();
-      SUnit &u = s->units[s->active[idx]];
-      if (u.next >= u.ks.size()) continue;
-      const int32_t K = u.ks[u.next];
+    for (size_t c = 0; c < s->active.size() && (int)r.jobs.size() < s->R; c++) {
+      const int64_t t = s->active[(s->rr + c) % s->active.size()];
+      SUnit &u = s->units[t];
+      const size_t at = u.next + taken[t];
+      if (at >= u.ks.size()) continue;
+      const int32_t K = u.ks[at];
       const int64_t need = host_surv_budget(K);
-      if (n > 0 && slots + need > b->surv_cap) {
+      if (!r.jobs.empty() && r.slots + need > b->surv_cap) {
         full = true;
         break;
       }
-      slots += need;
-      const int it0 = (u.next == 0 && K == u.r.kmin) ? u.r.iter_start : 0;
-      u.next++;
-      u.outstanding++;
-      s->h_jobs[n++] = make_job(K, u.r.shift, it0, u.slot);
-      owner.push_back(s->active[idx]);
+      r.slots += need;
+      const int it0 = (at == 0 && K == u.r.kmin) ? u.r.iter_start : 0;
+      r.jobs.push_back(make_job(K, u.r.shift, it0, u.slot));
+      r.owner.push_back(t);
+      taken[t]++;
       progress = true;
     }
   }
+  if (r.jobs.empty()) return 0;
   s->rr = s->active.empty() ? 0 : (s->rr + 1) % s->active.size();
-  if (n == 0) return AP26_OK;
-  s->h_rs->base = 0;
-  s->h_rs->njobs = n;
-  s->h_rs->per_round = s->R;
-  s->h_rs->pad = 0;
-  int rc = ensure_surv_cap(ctx, b, slots);
-  if (!rc) rc = ensure_graph(ctx, b);
+  const int rc = server_launch(s, r);
   if (rc) return rc;
-  HIPCHK(hipMemcpyAsync(b->d_jobs, s->h_jobs, sizeof(BatchJobDev) * n, hipMemcpyHostToDevice, ctx->stream));
-  HIPCHK(hipMemcpyAsync(b->d_rs, s->h_rs, sizeof(RoundState), hipMemcpyHostToDevice, ctx->stream));
-  rc = launch_round(ctx, b, nullptr, nullptr);
-  if (rc) return rc;
-  hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream,
-                     b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
-  hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream,
-                     b->d_sols, (const unsigned int *)(b->d_cnt + 2), BATCH_SOL_CAP);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(s->h_cnt, b->d_cnt, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, ctx->stream));
-  HIPCHK(hipMemcpyAsync(s->h_hash, b->d_hash, sizeof(unsigned long long) * s->max_units, hipMemcpyDeviceToHost,
-                        ctx->stream));
-  HIPCHK(hipStreamSynchronize(ctx->stream));
-  s->rounds++;
-  s->k_done += n;
-  const unsigned int nsol = (unsigned int)(s->h_cnt[2] & 0xffffffffu);
-  if (s->h_cnt[4] != 0) return AP26_E_INTERNAL; /* survivors beyond the round's budget */
-  if (s->h_cnt[6] != 0) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
+  for (const auto &kv : taken) {
+    SUnit &u = s->units[kv.first];
+    u.next += kv.second;
+    u.outstanding += (int)kv.second;
+  }
+  s->inflight.push_back(std::move(r));
+  s->next_io ^= 1;
+  return 1;
+}
+
+static void server_finish_unit(ap26_server *s, int64_t t, SUnit &u, double when) {
+  u.done = true;
+  u.t_done = when;
+  s->free_slots.push_back(u.slot);
+  u.slot = -1;
+  s->units_done++;
+  for (size_t i = 0; i < s->active.size(); i++)
+    if (s->active[i] == t) {
+      s->active.erase(s->active.begin() + (long)i);
+      break;
+    }
+  if (s->rr >= s->active.size()) s->rr = 0;
+}
+
+/* Wait for the oldest round in flight and hand its results to its units. */
+static int server_harvest(ap26_server *s, int32_t *completed) {
+  ap26_ctx *ctx = s->ctx;
+  BatchState *b = s->b;
+  SRound &r = s->inflight.front();
+  SPinned &p = s->pin[r.io];
+  HIPCHK(hipEventSynchronize(p.done));
+  if (p.cnt[6] != 0) return AP26_E_INTERNAL; /* a queue step ran on part of a wave */
+  if (p.cnt[4] != 0) {
+    /* survivors beyond the buffer: grow it to the count seen and run the
+       round again, after the other round in flight (if any) */
+    if (r.rerun) return AP26_E_INTERNAL;
+    r.rerun = true;
+    const int64_t need = (int64_t)(p.cnt[4] + (p.cnt[4] >> 3)) + b->surv_slack;
+    r.slots = need > r.slots ? need : r.slots;
+    int rc = server_launch(s, r);
+    if (rc) return rc;
+    HIPCHK(hipEventSynchronize(p.done));
+    if (p.cnt[6] != 0 || p.cnt[4] != 0) return AP26_E_INTERNAL;
+  }
+  const unsigned int nsol = (unsigned int)(p.cnt[2] & 0xffffffffu);
   if (nsol > (unsigned int)BATCH_SOL_CAP) return AP26_E_INTERNAL;
-  std::vector<DevSol> hs(nsol);
-  if (nsol) HIPCHK(hipMemcpy(hs.data(), b->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
+  std::vector<DevSol> extra;
+  const DevSol *sols = p.sols;
+  if (nsol > (unsigned int)SERVER_SOL_COPY) { /* rare: copy the rest, after the round behind it */
+    extra.resize(nsol);
+    HIPCHK(hipMemcpyAsync(extra.data(), b->io[r.io].d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    sols = extra.data();
+  }
   std::map<int, int64_t> slot_owner;
-  for (int64_t t : owner) slot_owner[s->units[t].slot] = t;
-  for (const DevSol &d : hs) {
+  std::map<int64_t, int> per_unit;
+  for (int64_t t : r.owner) {
+    slot_owner[s->units[t].slot] = t;
+    per_unit[t]++;
+  }
+  for (unsigned int i = 0; i < nsol; i++) {
+    const DevSol &d = sols[i];
     auto it = slot_owner.find(d.range_id);
     if (it == slot_owner.end()) return AP26_E_INTERNAL;
-    ap26_sol o{d.k, d.K, d.first, -1, d.shift_iter, d.prp2, d.valid};
-    s->units[it->second].sols.push_back(o);
+    SUnit &u = s->units[it->second];
+    if (u.status == AP26_OK) u.sols.push_back(ap26_sol{d.k, d.K, d.first, -1, d.shift_iter, d.prp2, d.valid});
   }
-  std::vector<int> cleared;
-  const double t = now_ms();
-  for (int64_t tk : owner) {
-    SUnit &u = s->units[tk];
-    u.outstanding--;
+  const double when = now_ms();
+  for (const auto &kv : per_unit) {
+    SUnit &u = s->units[kv.first];
+    u.outstanding -= kv.second;
+    u.hash += (int64_t)p.hash[u.slot];
+    if (p.big[u.slot] && u.status == AP26_OK) {
+      /* the walk reached 2^62, where the reference's PrimeQ exits: the
+         unit fails with AP26_E_RANGE and is not scheduled further */
+      u.status = AP26_E_RANGE;
+      u.next = u.ks.size();
+      u.sols.clear();
+    }
     if (u.outstanding == 0 && u.next >= u.ks.size() && !u.done) {
-      u.done = true;
-      u.t_done = t;
-      u.hash = (int64_t)s->h_hash[u.slot];
-      cleared.push_back(u.slot);
-      s->free_slots.push_back(u.slot);
-      u.slot = -1;
-      s->units_done++;
+      server_finish_unit(s, kv.first, u, when);
       if (completed) (*completed)++;
-      for (size_t i = 0; i < s->active.size(); i++)
-        if (s->active[i] == tk) {
-          s->active.erase(s->active.begin() + (long)i);
-          break;
-        }
     }
   }
-  if (s->rr >= s->active.size()) s->rr = 0;
-  /* clear the round counters (sols, tests) and the finished units' totals */
-  for (int slot : cleared)
-    hipLaunchKernelGGL(server_zero_slot_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_hash, slot);
-  hipLaunchKernelGGL(server_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, b->d_cnt);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(ctx->stream));
+  s->rounds++;
+  s->k_done += (int64_t)r.jobs.size();
+  s->inflight.pop_front();
   return AP26_OK;
+}
+
+extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
+  if (!s) return AP26_E_RANGE;
+  ap26_ctx *ctx = s->ctx;
+  if (completed) *completed = 0;
+  HIPCHK(hipSetDevice(ctx->device));
+  while (s->inflight.size() < 2) {
+    const int rc = server_enqueue(s);
+    if (rc < 0) return rc;
+    if (rc == 0) break;
+  }
+  if (s->inflight.empty()) return AP26_OK;
+  return server_harvest(s, completed);
 }
 
 extern "C" int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out, int32_t cap, int32_t *n_out,
@@ -801,10 +964,17 @@ extern "C" int ap26_server_result(ap26_server *s, int64_t ticket, ap26_sol *out,
   SUnit &u = it->second;
   *done = u.done ? 1 : 0;
   if (!u.done) return AP26_OK;
+  if (latency_ms) *latency_ms = u.t_done - u.t_submit;
+  if (u.status != AP26_OK) { /* collected: the unit failed */
+    const int st = u.status;
+    if (n_out) *n_out = 0;
+    if (hash_total) *hash_total = 0;
+    s->units.erase(it);
+    return st;
+  }
   const int n = (int)u.sols.size();
   if (n_out) *n_out = n;
   if (hash_total) *hash_total = u.hash;
-  if (latency_ms) *latency_ms = u.t_done - u.t_submit;
   if (n > cap) return AP26_E_OVERFLOW; /* kept: call again with a larger buffer */
   for (int i = 0; i < n; i++) out[i] = u.sols[i];
   s->units.erase(it);
@@ -848,7 +1018,7 @@ static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,
                              ap26_tune_entry *results, int32_t cap, int32_t *n_out, ap26_tune_entry *best) {
-  if (!ctx || K < 1 || K > 600000000 || !host_will_search(K) || !host_shift_ok(K, SHIFT)) return AP26_E_RANGE;
+  if (!ctx || K < 1 || K > 600000000 || !host_will_search(K) || !host_walk_ok(K, SHIFT)) return AP26_E_RANGE;
   HIPCHK(hipSetDevice(ctx->device));
   const int nv = quick ? 5 : (int)(sizeof(TUNE_VARIANTS) / sizeof(int));
   const int nc = quick ? 2 : (int)(sizeof(TUNE_CHUNKS) / sizeof(int));
@@ -864,7 +1034,7 @@ extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t qu
       ctx->tune_variant = TUNE_VARIANTS[vi];
       ctx->tune_chunk = quick ? TUNE_CHUNKS[ci + 1] : TUNE_CHUNKS[ci];
       BatchState *tmp = nullptr;
-      rc = batch_state_new(ctx, 1, &tmp);
+      rc = batch_state_new(ctx, 1, 1, &tmp);
       if (rc) break;
       ctx->batch = tmp;
       int32_t ns = 0;

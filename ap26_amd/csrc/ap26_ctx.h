@@ -27,7 +27,7 @@ using namespace ap26;
 constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* initial survivor buffer: 67M (~40M per K) */
 constexpr int SOL_CAP = 1 << 16;
 constexpr int NCOUNT = 8; /* device counters */
-enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_LIVE = 3, C_ERR = 7 };
+enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_LIVE = 3, C_BIG = 4, C_ERR = 7 };
 
 using SieveFn = void (*)(SieveArgs);
 

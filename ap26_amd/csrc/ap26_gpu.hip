@@ -242,9 +242,11 @@ static int ensure_surv(ap26_ctx *ctx, int64_t need) {
   return AP26_OK;
 }
 
-static bool span_ok(int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1) {
+/* walk: the span is searched (walk and reports), not only sieved */
+static bool span_ok(int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1, bool walk) {
   if (K < 1 || K > 600000000) return false;       /* per-K constants exact below ~6.3e8 */
   if (!host_shift_ok(K, SHIFT)) return false;   /* every term fits in int64 */
+  if (walk && !host_walk_ok(K, SHIFT)) return false; /* the reference's 2^62 PrimeQ limit */
   if (it0 < 0 || it1 > AP26_NSHIFT || it0 > it1) return false;
   if (j0 < 0 || j1 > AP26_NN43 || j0 > j1) return false;
   return true;
@@ -255,7 +257,7 @@ extern "C" {
 int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0,
                      int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out, ap26_stats *stats) {
   if (!ctx || !n_out || (cap > 0 && !out)) return AP26_E_RANGE;
-  if (!span_ok(K, SHIFT, it0, it1, j0, j1)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, it0, it1, j0, j1, true)) return AP26_E_RANGE;
   auto t0 = std::chrono::steady_clock::now();
   HIPCHK(hipSetDevice(ctx->device));
   ap26_stats st{};
@@ -289,6 +291,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.ntests = ctx->d_cnt + C_NTESTS;
     w.err = (unsigned int *)(ctx->d_cnt + C_ERR);
     w.nlive = ctx->d_cnt + C_LIVE;
+    w.big = ctx->d_cnt + C_BIG;
     hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
@@ -308,6 +311,11 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
   }
   const unsigned int nsol = (unsigned int)(cnt[C_NSOL] & 0xffffffffu);
   if (nsol > (unsigned int)SOL_CAP) return AP26_E_INTERNAL;
+  if (cnt[C_BIG]) { /* the walk reached 2^62, where the reference's PrimeQ exits */
+    st.walk_past_limit = (int64_t)cnt[C_BIG];
+    if (stats) *stats = st;
+    return AP26_E_RANGE;
+  }
   std::vector<DevSol> hs(nsol);
   if (nsol) {
     HIPCHK(hipMemcpy(hs.data(), ctx->d_sols, sizeof(DevSol) * nsol, hipMemcpyDeviceToHost));
@@ -343,7 +351,7 @@ int ap26_search_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, a
 int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0, int32_t j1,
                     int64_t *out, int64_t cap, int64_t *n_out) {
   if (!ctx || !n_out || (cap > 0 && !out)) return AP26_E_RANGE;
-  if (!span_ok(K, SHIFT, it0, it1, j0, j1)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, it0, it1, j0, j1, false)) return AP26_E_RANGE;
   HIPCHK(hipSetDevice(ctx->device));
   *n_out = 0;
   if (it0 == it1 || j0 == j1) return AP26_OK;
@@ -410,7 +418,7 @@ int ap26_verify_sols(ap26_ctx *ctx, ap26_sol *sols, int32_t n) {
 
 int ap26_okok_table(ap26_ctx *ctx, int32_t K, int32_t SHIFT, uint64_t *out) {
   if (!ctx || !out) return AP26_E_RANGE;
-  if (!span_ok(K, SHIFT, 0, 1, 0, 1)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, 0, 1, 0, 1, false)) return AP26_E_RANGE;
   HIPCHK(hipSetDevice(ctx->device));
   KParams kp;
   HostK hk;

@@ -91,6 +91,21 @@ static inline int host_shift_ok(int32_t K, int32_t SHIFT) {
   return top < ((__int128)1 << 63);
 }
 
+/* The reference CPU build tests every walked term with PrimeQ, which exits
+   on any N >= 2^62 (src/common/PrimeQ_x86.h:160-166, its x87 multiply
+   limit).  A survivor's walk starts at survivor + 5*STEP; when that is past
+   2^62 for the largest candidates, (SHIFT + 640)*MOD - 1 + 5*STEP >= 2^62,
+   the reference stops at the first of them, so the search refuses such a K
+   before any GPU work.  Below that bound, whether a walk reaches 2^62 depends
+   on the data: the device counts such terms and the call fails with
+   AP26_E_RANGE (the reference's exit) instead of reporting. */
+#define AP26_WALK_LIMIT_BITS 62
+static inline int host_walk_ok(int32_t K, int32_t SHIFT) {
+  if (!host_shift_ok(K, SHIFT)) return 0;
+  const __int128 first = ((__int128)SHIFT + 640) * AP26_MOD - 1 + (__int128)5 * K * AP26_PRIM23;
+  return first < ((__int128)1 << AP26_WALK_LIMIT_BITS);
+}
+
 /* Survivor-buffer entries to hold for one K.  A sieve prime p (61..541)
    rejects 24 of its p residues, or only residue 0 when p divides K (then
    STEP = 0 mod p), so each such prime scales the expected survivor count by

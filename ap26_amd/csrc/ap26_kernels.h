@@ -1247,7 +1247,12 @@ struct WEntry {
 
 constexpr int WALK_QCAP = 128;
 
-template <class StepOf, class Close>
+/* Walked terms at or past 2^62: the reference CPU build's PrimeQ exits on
+   the first (src/common/PrimeQ_x86.h:160-166).  Only forward terms matter:
+   the backward extension tests terms below the run's first forward term. */
+constexpr uint64_t WALK_LIMIT = UINT64_C(1) << 62;
+
+template <class StepOf, class Close, class Big>
 struct WalkQueue {
   WEntry *q; /* this wave's WALK_QCAP entries */
   int qn = 0;
@@ -1256,6 +1261,7 @@ struct WalkQueue {
   unsigned long long tests = 0;
   StepOf step_of;
   Close close;
+  Big big; /* called with the job of a walked term >= WALK_LIMIT */
 
   __device__ __forceinline__ void push(const WEntry &e, bool pred) {
     const uint64_t bal = __ballot(pred);
@@ -1271,6 +1277,7 @@ struct WalkQueue {
     bool go = false;
     if (pred) {
       tests++;
+      if ((uint64_t)e.m >= WALK_LIMIT) big(e.job);
       go = sprp2((uint64_t)e.m);
       if (go) {
         e.k++;
@@ -1333,6 +1340,7 @@ struct BatchWalkArgs {
   unsigned long long *__restrict__ overflow; /* largest survivor count beyond cap */
   unsigned long long *__restrict__ nlive;    /* survivors walked, over all rounds */
   unsigned int *__restrict__ err;            /* a queue step ran on part of a wave */
+  unsigned long long *__restrict__ big;      /* per range: walked terms >= 2^62 */
 };
 
 static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a) {
@@ -1364,8 +1372,9 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
       }
     }
   };
-  WalkQueue<decltype(step_of), decltype(close)> wq{s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0,
-                                                   step_of, close};
+  auto big = [&](int32_t job) { atomicAdd(&a.big[jobs[job].range_id], 1ull); };
+  WalkQueue<decltype(step_of), decltype(close), decltype(big)> wq{
+      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
   unsigned long long live = 0;
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
@@ -1416,6 +1425,7 @@ struct WalkArgs {
   unsigned long long *__restrict__ ntests;
   unsigned int *__restrict__ err; /* a queue step ran on part of a wave */
   unsigned long long *__restrict__ nlive; /* survivors walked */
+  unsigned long long *__restrict__ big;   /* walked terms >= 2^62 */
 };
 
 static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
@@ -1441,8 +1451,10 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
       }
     }
   };
-  WalkQueue<decltype(step_of), decltype(close)> wq{s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0,
-                                                   step_of, close};
+  unsigned long long nbig = 0;
+  auto big = [&](int32_t) { nbig++; };
+  WalkQueue<decltype(step_of), decltype(close), decltype(big)> wq{
+      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
   unsigned long long live = 0;
@@ -1463,6 +1475,7 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
   if (wq.tests) atomicAdd(a.ntests, wq.tests);
   if (live) atomicAdd(a.nlive, live);
+  if (nbig) atomicAdd(a.big, nbig);
 }
 
 /* ---- batched SPRP ------------------------------------------------------ */

@@ -55,7 +55,8 @@ class Stats(ctypes.Structure):
                 ("prp_tests", ctypes.c_int64), ("reports", ctypes.c_int64),
                 ("ms_tables", ctypes.c_double), ("ms_sieve", ctypes.c_double),
                 ("ms_walk", ctypes.c_double), ("ms_total", ctypes.c_double),
-                ("sieve_launches", ctypes.c_int32), ("walk_launches", ctypes.c_int32)]
+                ("sieve_launches", ctypes.c_int32), ("walk_launches", ctypes.c_int32),
+                ("walk_past_limit", ctypes.c_int64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -128,14 +129,19 @@ def load(path=LIB_PATH):
 
 
 class Ap26Error(RuntimeError):
-    def __init__(self, code, what):
+    """A negative AP26_E_* return; `stats` holds the call's ap26_stats when the
+    entry point fills them on failure (AP26_E_RANGE from a walk that reached
+    2^62 carries stats.walk_past_limit)."""
+
+    def __init__(self, code, what, stats=None):
         self.code = code
+        self.stats = stats
         super().__init__(f"{what}: {load().ap26_strerror(code).decode()} ({code})")
 
 
-def _check(code, what):
+def _check(code, what, stats=None):
     if code != AP26_OK:
-        raise Ap26Error(code, what)
+        raise Ap26Error(code, what, stats)
 
 
 def _ptr(a):
@@ -186,7 +192,7 @@ class Context:
         st = Stats()
         rc = self._lib.ap26_search_k(self.h, K, shift, iter_start, _ptr(out), cap,
                                      ctypes.byref(n), ctypes.byref(st))
-        _check(rc, "ap26_search_k")
+        _check(rc, "ap26_search_k", st)
         return out[:n.value].copy(), st
 
     def search_span(self, K, shift, it0, it1, j0, j1, cap=1 << 14):
@@ -195,7 +201,7 @@ class Context:
         st = Stats()
         rc = self._lib.ap26_search_span(self.h, K, shift, it0, it1, j0, j1, _ptr(out), cap,
                                         ctypes.byref(n), ctypes.byref(st))
-        _check(rc, "ap26_search_span")
+        _check(rc, "ap26_search_span", st)
         return out[:n.value].copy(), st
 
     def sieve_span(self, K, shift, it0, it1, j0, j1, cap=1 << 22):
@@ -240,7 +246,7 @@ class Context:
         st = Stats()
         rc = self._lib.ap26_search_batch(self.h, _ptr(r), len(r), _ptr(out), cap, ctypes.byref(n),
                                          _ptr(tot), ctypes.byref(st))
-        _check(rc, "ap26_search_batch")
+        _check(rc, "ap26_search_batch", st)
         return out[:n.value].copy(), tot, st
 
     def autotune(self, K=48620637, shift=640, quick=True, cache_path=None):
@@ -318,7 +324,8 @@ class Server:
         return n.value
 
     def result(self, ticket, cap=1 << 16):
-        """None while running, else (reports, hash_total, latency_ms)."""
+        """None while running, else (reports, hash_total, latency_ms); a unit
+        whose walk reached 2^62 raises Ap26Error(AP26_E_RANGE) once done."""
         out = np.zeros(cap, dtype=SOL_DTYPE)
         n = ctypes.c_int32()
         h = ctypes.c_int64()

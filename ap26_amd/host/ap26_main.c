@@ -7,8 +7,10 @@
  * Options (the reference's -compute and CPU-ISA flags are accepted):
  *   -device N      HIP device (default 0)
  *   -report N      SOL threshold, the reference's MINIMUM_AP_LENGTH_TO_REPORT (20)
- *   -shiftckpt     checkpoint after every shift (ITER = shifts done), as the
- *                  reference CPU build does; default is one checkpoint per K
+ *   -shiftckpt     checkpoint inside each K at ITER 4 and 8, the reference
+ *                  AVX2 build's cadence (src/cpu/cpuavx2.cpp:858-861); the
+ *                  spans [0,4), [4,8), [8,10) keep the shift pairs of the
+ *                  sieve whole; default is one checkpoint per K
  *   -batch         search the unit with the batched persistent engine, 256
  *                  searched K per call (AP26_BATCH_K_PER_CALL overrides); the
  *                  state files, records and hash line are those of the per-K
@@ -67,6 +69,39 @@ static int cmp_sol_unit(const void *a, const void *b) {
   return cmp_sol(a, b);
 }
 
+/* The reference CPU build's PrimeQ exits on a walked term >= 2^62
+   (src/common/PrimeQ_x86.h:160-166); the library reports that case as
+   AP26_E_RANGE and the command exits with the reference's message. */
+static void walk_limit_message(int32_t K, int32_t SHIFT) {
+  fprintf(stderr, "PrimeQ: modulus out of range.\n");
+  fprintf(stderr, "Error: K %d at SHIFT %d walks terms at or past 2^62 (the reference's PrimeQ limit)\n", K,
+          SHIFT);
+}
+
+/* One K of a unit from shift iteration ITER: checkpoint, search, report.
+   Returns 0, AP26_E_RANGE (the walk reached 2^62: the reference would exit
+   here), another AP26_E_* code, or a host reporting error. */
+static int search_one_k(ap26_ctx *ctx, ap26_host *h, int32_t K, int32_t ITER, int shiftckpt, ap26_sol *sols) {
+  ap26h_checkpoint(h, K, 0, ITER);
+  int rc;
+  if (shiftckpt) {
+    for (int it = ITER; it < 10;) {
+      const int it1 = it < 8 ? (it / 4 + 1) * 4 : 10;
+      int32_t n = 0;
+      rc = ap26_search_span(ctx, K, h->SHIFT, it, it1, 0, 10840, sols, SOL_CAP, &n, NULL);
+      if (!rc) rc = report_all(h, sols, n);
+      if (rc) return rc;
+      if (it1 < 10) ap26h_checkpoint(h, K, 0, it1);
+      it = it1;
+    }
+    return 0;
+  }
+  int32_t n = 0;
+  rc = ap26_search_k(ctx, K, h->SHIFT, ITER, sols, SOL_CAP, &n, NULL);
+  if (!rc) rc = report_all(h, sols, n);
+  return rc;
+}
+
 /* Open a unit's state as AP26.cpp:789-832 does: resume (K, ITER) from the
    newest state file, or start at KMIN. */
 static void unit_begin(ap26_host *h, int32_t *K, int32_t *ITER) {
@@ -98,6 +133,17 @@ typedef struct {
   int32_t K, ITER, Kend;
   int done;
 } unit_t;
+
+/* last checkpoint and hash line of a finished manifest unit */
+static int finish_unit(unit_t *x) {
+  int status = EXIT_SUCCESS;
+  ap26h_checkpoint(&x->h, x->K, 1, 0);
+  if (ap26h_write_hash(&x->h)) status = EXIT_FAILURE;
+  else fprintf(stderr, "Workunit %d %d %d complete.\n", x->h.KMIN, x->h.KMAX, x->h.SHIFT);
+  ap26h_close(&x->h);
+  x->done = 1;
+  return status;
+}
 
 #define MANIFEST_SOL_CAP (1 << 20)
 #define MANIFEST_K_PER_UNIT 16  /* searched K per unit per batched call */
@@ -156,6 +202,7 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
   int *owner = status == EXIT_SUCCESS ? (int *)malloc(sizeof(int) * (size_t)(nu + 1)) : NULL;
   if (status == EXIT_SUCCESS && (!sols || !r || !owner)) status = EXIT_FAILURE;
   printf("%d work units in %s\n", nu, path);
+  int failed = 0; /* a unit stopped at the reference's 2^62 limit */
   while (status == EXIT_SUCCESS) {
     int nr = 0, kcall = 0;
     for (int i = 0; i < nu && kcall < MANIFEST_K_PER_CALL; i++) {
@@ -191,6 +238,34 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
     if (status != EXIT_SUCCESS || nr == 0) break;
     int32_t n = 0;
     int rc = ap26_search_batch(ctx, r, nr, sols, MANIFEST_SOL_CAP, &n, NULL, NULL);
+    if (rc == AP26_E_RANGE) {
+      /* a walk of some unit reached 2^62: search this call's units K by K,
+         so each stops where its own `AP26 KMIN KMAX SHIFT` run would */
+      for (int ri = 0; ri < nr && status == EXIT_SUCCESS; ri++) {
+        unit_t *x = &u[owner[ri]];
+        int urc = 0;
+        for (int32_t K = x->K; K <= x->Kend && r[ri].kmax >= r[ri].kmin && !urc; K++) {
+          if (!ap26h_will_search(K)) continue;
+          urc = search_one_k(ctx, &x->h, K, x->ITER, 0, sols);
+          if (urc == AP26_E_RANGE) walk_limit_message(K, x->h.SHIFT);
+          if (!urc) {
+            x->ITER = 0;
+            x->h.K_DONE++;
+          }
+        }
+        if (urc) {
+          fprintf(stderr, "Workunit %d %d %d stopped.\n", x->h.KMIN, x->h.KMAX, x->h.SHIFT);
+          ap26h_close(&x->h);
+          x->done = 1;
+          failed = 1;
+          if (urc != AP26_E_RANGE) status = EXIT_FAILURE;
+          continue;
+        }
+        x->K = x->Kend + 1;
+        if (x->K > x->h.KMAX) status = finish_unit(x);
+      }
+      continue;
+    }
     if (rc) {
       fprintf(stderr, "Error: GPU search failed: %s\n", ap26_strerror(rc));
       status = EXIT_FAILURE;
@@ -217,13 +292,7 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
         x->h.K_DONE++;
       }
       x->K = x->Kend + 1;
-      if (status == EXIT_SUCCESS && x->K > x->h.KMAX) {
-        ap26h_checkpoint(&x->h, x->K, 1, 0);
-        if (ap26h_write_hash(&x->h)) status = EXIT_FAILURE;
-        else fprintf(stderr, "Workunit %d %d %d complete.\n", x->h.KMIN, x->h.KMAX, x->h.SHIFT);
-        ap26h_close(&x->h);
-        x->done = 1;
-      }
+      if (status == EXIT_SUCCESS && x->K > x->h.KMAX) status = finish_unit(x);
     }
   }
   for (int i = 0; i < nu; i++)
@@ -232,7 +301,7 @@ static int run_manifest(ap26_ctx *ctx, const char *path, int32_t min_report, int
   free(r);
   free(owner);
   free(u);
-  return status;
+  return failed ? EXIT_FAILURE : status;
 }
 
 int main(int argc, char *argv[]) {
@@ -349,6 +418,12 @@ int main(int argc, char *argv[]) {
         per_call = (per_call + 1) / 2;
         continue;
       }
+      if (rc == AP26_E_RANGE) {
+        /* a walk in this chunk reached 2^62: go on K by K (below), which
+           stops at the K where the reference would */
+        batch = 0;
+        break;
+      }
       if (rc) {
         fprintf(stderr, "Error: GPU search failed: %s\n", ap26_strerror(rc));
         status = EXIT_FAILURE;
@@ -376,23 +451,16 @@ int main(int argc, char *argv[]) {
       }
       K = Kend + 1;
     }
-  } else {
+  }
+  if (!batch) {
     for (; K <= KMAX && status == EXIT_SUCCESS; ++K) {
       if (!ap26h_will_search(K)) continue;
-      ap26h_checkpoint(&h, K, 0, ITER);
       time_t t0 = time(NULL);
-      if (shiftckpt) {
-        for (int it = ITER; it < 10; it++) {
-          int32_t n = 0;
-          rc = ap26_search_span(ctx, K, SHIFT, it, it + 1, 0, 10840, sols, SOL_CAP, &n, NULL);
-          if (rc) break;
-          if ((rc = report_all(&h, sols, n)) != 0) break;
-          if (it + 1 < 10) ap26h_checkpoint(&h, K, 0, it + 1);
-        }
-      } else {
-        int32_t n = 0;
-        rc = ap26_search_k(ctx, K, SHIFT, ITER, sols, SOL_CAP, &n, NULL);
-        if (!rc) rc = report_all(&h, sols, n);
+      rc = search_one_k(ctx, &h, K, ITER, shiftckpt, sols);
+      if (rc == AP26_E_RANGE) {
+        walk_limit_message(K, SHIFT);
+        status = EXIT_FAILURE;
+        break;
       }
       if (rc) {
         fprintf(stderr, "Error: search of K %d failed: %s\n", K,

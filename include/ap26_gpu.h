@@ -69,13 +69,20 @@ typedef struct {
   int64_t prp_tests;        /* base-2 SPRP tests run by the walk */
   int64_t reports;          /* (k >= 10) reports */
   double ms_tables;         /* table-build kernel time */
-  double ms_sieve;          /* sieve kernel time (batch: the last round's
-                               persistent sieve launch) */
+  double ms_sieve;          /* sieve kernel time; batch: the sum over the
+                               timed rounds (the first 1024 of each segment)
+                               of the round's sieve graph (n43 starts, tables,
+                               persistent sieve), sieve_launches = the number
+                               of rounds timed */
   double ms_walk;           /* walk kernel time (batch: device time of all
                                rounds) */
   double ms_total;          /* wall time of the call */
   int32_t sieve_launches;
   int32_t walk_launches;
+  int64_t walk_past_limit;  /* walked terms >= 2^62: the reference CPU build's
+                               PrimeQ exits on the first
+                               (src/common/PrimeQ_x86.h:160-166); a call that
+                               meets one returns AP26_E_RANGE and no reports */
 } ap26_stats;
 
 typedef struct ap26_ctx ap26_ctx;

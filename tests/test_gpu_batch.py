@@ -106,12 +106,52 @@ def test_batch_edge_cases(gpu_ctx):
         gpu_ctx.search_batch([(5, 4, 0, 0), (0, 3, 0, 0)])
     with pytest.raises(gpu.Ap26Error):  # terms past 2^63
         gpu_ctx.search_batch([(366384, 366384, 36000, 0)])
-    # the largest K and SHIFT the bound admits (terms up to 8.7e18 < 2^63):
-    # batched equals plain, survivors and reports
-    sols, tot, st = gpu_ctx.search_batch([(600000000, 600000000, 16912, 0)])
-    psols, pst = gpu_ctx.search_k(600000000, 16912)
-    assert st.survivors == pst.survivors > 30_000_000
+    # the largest K at a large SHIFT whose walks stay below 2^62: batched
+    # equals plain, survivors and reports
+    sols, tot, st = gpu_ctx.search_batch([(600000000, 600000000, 8000, 0)])
+    psols, pst = gpu_ctx.search_k(600000000, 8000)
+    assert st.survivors == pst.survivors > 30_000_000 and st.walk_past_limit == pst.walk_past_limit == 0
     assert key(sols) == key(psols)
+
+
+def test_walk_limit_of_the_reference(gpu_ctx):
+    """The reference CPU build's PrimeQ exits on a walked term >= 2^62
+    (src/common/PrimeQ_x86.h:160-166).  K = 6e8 at SHIFT 16912: the first
+    walked term of the largest candidates is past 2^62, so the search is
+    refused before any GPU work.  At SHIFT 14000 it is not, but walks reach
+    2^62 at their third term: both paths run the K, count the same walked
+    terms past 2^62 and fail with AP26_E_RANGE instead of reporting."""
+    for call in (lambda: gpu_ctx.search_k(600000000, 16912),
+                 lambda: gpu_ctx.search_batch([(600000000, 600000000, 16912, 0)])):
+        with pytest.raises(gpu.Ap26Error) as e:
+            call()
+        assert e.value.code == gpu.AP26_E_RANGE
+    with pytest.raises(gpu.Ap26Error) as ep:
+        gpu_ctx.search_k(600000000, 14000)
+    with pytest.raises(gpu.Ap26Error) as eb:
+        gpu_ctx.search_batch([(600000000, 600000000, 14000, 0), (366384, 366384, 0, 0)])
+    assert ep.value.code == eb.value.code == gpu.AP26_E_RANGE
+    assert ep.value.stats.walk_past_limit == eb.value.stats.walk_past_limit > 0
+    # the same unit in the server fails alone; its neighbour completes
+    with gpu.Server(gpu_ctx, max_units=4, mem_budget_bytes=3 << 30) as srv:
+        bad = srv.submit(600000000, 600000000, 14000)
+        good = srv.submit(366384, 366384, 0)
+        with pytest.raises(gpu.Ap26Error):  # refused at admission
+            srv.submit(600000000, 600000000, 16912)
+        res, err = None, None
+        while res is None or err is None:
+            srv.pump()
+            if res is None:
+                res = srv.result(good)
+            if err is None:
+                try:
+                    if srv.result(bad) is not None:
+                        err = "completed"
+                except gpu.Ap26Error as x:
+                    err = x.code
+        assert err == gpu.AP26_E_RANGE
+        want = plain_reports(gpu_ctx, 366384, 366384, 0)
+        assert key(res[0]) == want and res[1] == sum(f % 1000 + k for k, _, f in want)
 
 
 def test_server_alone_and_in_mix(gpu_ctx):

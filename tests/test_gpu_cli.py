@@ -66,15 +66,17 @@ def test_resume_from_reference_style_checkpoint(golden_index, tmp_path):
 
 
 def test_resume_mid_k_and_shift_checkpoints(golden_index, tmp_path):
-    """-shiftckpt writes ITER = shifts done, as the reference CPU build does; a
-    run resumed from its ITER=4 state ends with the golden hash."""
+    """-shiftckpt checkpoints at ITER 4 and 8 inside each K: the Checkpoint
+    lines equal those the reference AVX2 build printed for the same unit; a
+    run resumed from an ITER=4 state ends with the golden hash."""
     a = tmp_path / "a"
     a.mkdir()
     r = run([875861, 875861, 640, "-report", 10, "-shiftckpt"], a)
     assert r.returncode == 0, r.stderr
     ck = [l for l in r.stdout.splitlines() if l.startswith("Checkpoint:")]
-    assert [l.split("ITER:")[1] for l in ck] == [str(i) for i in range(10)] + ["0"]
     g = golden(golden_index, "c1_875861")
+    assert ck == g["checkpoints"]
+    assert [l.split("ITER:")[1] for l in ck] == ["0", "4", "8", "0"]
     assert read_sol(str(a / "SOL-AP26.txt")) == read_sol(os.path.join(GOLDEN, "ref_min10", "c1_875861.sol"))
     # replay: take the hash the run had after 4 shifts, resume from there
     b = tmp_path / "b"
@@ -208,3 +210,28 @@ def test_metrics_records(tmp_path):
     for d in ("unit_875858_875862_640", "unit_366384_366384_0"):
         recs = [json.loads(l) for l in open(m / d / "m.jsonl")]
         assert recs[-1]["progress"] == 1.0 and recs[-1]["k_done"] == recs[-1]["k_count"]
+
+
+def test_walk_limit_exits_like_the_reference(tmp_path):
+    """The reference CPU build exits (status 1, "modulus out of range") when
+    a walked term reaches 2^62 (src/common/PrimeQ_x86.h:160-166).  K = 6e8
+    at SHIFT 16912 is refused before the search; at SHIFT 14000 the walks
+    reach 2^62 on the device; -batch stops at the same K with the same state
+    files as the per-K run."""
+    for shift in (16912, 14000):
+        d = tmp_path / str(shift)
+        d.mkdir()
+        r = run([600000000, 600000000, shift], d)
+        assert r.returncode == 1 and "modulus out of range" in r.stderr, r.stderr
+        assert "Workunit complete." not in r.stderr
+    a, b = tmp_path / "a", tmp_path / "b"
+    a.mkdir()
+    b.mkdir()
+    ra = run([599999998, 600000000, 14000], a)
+    rb = run([599999998, 600000000, 14000, "-batch"], b)
+    assert ra.returncode == rb.returncode == 1
+    for f in ("AP26-state.a.txt", "AP26-state.b.txt"):
+        fa, fb = a / f, b / f
+        assert fa.exists() == fb.exists()
+        if fa.exists():
+            assert fa.read_text().split()[:6] == fb.read_text().split()[:6]

@@ -68,7 +68,7 @@ def test_sieve_span_parity(variant, span):
 
 @pytest.mark.parametrize("span", [(366384, 0, 0, 10, 0, 20), (875862, 640, 0, 10, 2000, 2020),
                                   (44121555, 0, 0, 10, 10800, 10840), (61 * 67 * 71 * 73, 0, 0, 10, 100, 110),
-                                  (600000000, 16912, 0, 10, 0, 20)])
+                                  (600000000, 8000, 0, 10, 0, 20)])
 def test_search_span_parity(gpu_ctx, span):
     g, st = gpu_ctx.search_span(*span)
     o, nsurv = O.search_span(*span, threads=4)
@@ -203,7 +203,8 @@ def test_edge_cases(gpu_ctx):
     for args in [(0, 0, 0, 10, 0, 1), (366384, 0, 0, 11, 0, 1), (366384, 0, 0, 10, 0, 10841),
                  (366384, -64, 0, 10, 0, 1), (366384, 0, 5, 4, 0, 1),
                  (366384, 36000, 0, 10, 0, 1), (366384, 1 << 20, 0, 10, 0, 1),
-                 (600000000, 20000, 0, 10, 0, 1)]:  # terms past 2^63
+                 (600000000, 20000, 0, 10, 0, 1),  # terms past 2^63
+                 (600000000, 16912, 0, 10, 0, 1)]:  # first walked terms past 2^62
         rc = lib.ap26_search_span(gpu_ctx.h, *args, None, 0, ctypes.byref(n), None)
         assert rc == gpu.AP26_E_RANGE
     # capacity overflow reports the true count
