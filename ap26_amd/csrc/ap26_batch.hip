@@ -611,10 +611,12 @@ extern "C" int ap26_batch_sieve_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32
 /* ---- serving ------------------------------------------------------------ */
 /* Continuous batching.  Units are admitted by slots and by a device-memory
    budget (R K per round); each round takes the next K of every active unit
-   in turn.  Two rounds are in flight: pump() keeps a second round queued on
-   the stream behind the running one and only then waits for the older
-   round, so the device never idles while the host harvests a round, admits
-   new units and builds the next.  A round's outputs (counters, reports,
+   in turn.  Up to two rounds are in flight: when the units waiting fill a
+   whole second round, pump() queues it on the stream behind the running one
+   before it waits for the older round, so a saturated device never idles
+   while the host harvests a round, admits new units and builds the next;
+   below saturation it queues nothing behind the running round, so a unit
+   that arrives meanwhile joins the very next round.  A round's outputs (counters, reports,
    per-slot hash sums and 2^62 counts) live in its own RoundIO and are copied
    to pinned host buffers on the stream; per-unit hash totals are summed on
    the host from the per-round sums, so a round can be run again (a survivor
@@ -817,9 +819,11 @@ static int server_launch(ap26_server *s, const SRound &r) {
 /* Build the next round (one K from each active unit in turn until R jobs or
    the survivor slots are taken; a K divisible by sieve primes takes more
    slots, and one that needs more than the buffers hold alone grows them) and
-   launch it.  Returns 1 if a round was launched, 0 if no unit has K left,
-   or an error, in which case the units' scheduling is left as it was. */
-static int server_enqueue(ap26_server *s) {
+   launch it.  With require_full, only a round that is full (R jobs, or the
+   survivor slots taken) is launched.  Returns 1 if a round was launched, 0
+   if not, or an error, in which case the units' scheduling is left as it
+   was. */
+static int server_enqueue(ap26_server *s, bool require_full) {
   BatchState *b = s->b;
   SRound r;
   r.io = s->next_io;
@@ -847,7 +851,7 @@ static int server_enqueue(ap26_server *s) {
       progress = true;
     }
   }
-  if (r.jobs.empty()) return 0;
+  if (r.jobs.empty() || (require_full && !full && (int)r.jobs.size() < s->R)) return 0;
   s->rr = s->active.empty() ? 0 : (s->rr + 1) % s->active.size();
   const int rc = server_launch(s, r);
   if (rc) return rc;
@@ -948,7 +952,7 @@ extern "C" int ap26_server_pump(ap26_server *s, int32_t *completed) {
   if (completed) *completed = 0;
   HIPCHK(hipSetDevice(ctx->device));
   while (s->inflight.size() < 2) {
-    const int rc = server_enqueue(s);
+    const int rc = server_enqueue(s, !s->inflight.empty());
     if (rc < 0) return rc;
     if (rc == 0) break;
   }
