@@ -79,6 +79,8 @@ struct ap26_ctx {
   BatchState *batch = nullptr;
   int tune_variant = -1; /* batched sieve variant / work-item chunk chosen by */
   int tune_chunk = -1;   /* ap26_autotune (or its cache); -1 = default       */
+  void *scratch = nullptr; /* this device's address of the allocation-free entry
+                              point's scratch array (hipGetSymbolAddress) */
 };
 
 

@@ -4,9 +4,11 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -65,6 +67,49 @@ SieveVariant pick_variant(int id) {
   }
 }
 
+/* ---- scratch of the allocation-free entry point -------------------------- */
+/* ap26_search_span_noalloc works in one file-scope device array: the tables
+   of 10 shifts, the n43 starts, NOALLOC_SURV survivor slots, the counters and
+   NOALLOC_SOLS reports (about 34 MB, held on every device the library runs
+   on until the process ends).  Its survivor slots hold every survivor of
+   NOALLOC_MAX_N43 n43 starts of any K <= 6e8: at most 40.6 M per K x 5.45
+   (K = 61*67*71*73) x 64/10840, plus the slots the sieve's waves reserve and
+   leave unused (up to 256 CUs x 12 waves x 128 per shift-pair launch, 5
+   launches): 3.4 M < 2^22. */
+constexpr int NOALLOC_MAX_N43 = 64;
+constexpr int64_t NOALLOC_SURV = INT64_C(1) << 22;
+constexpr int NOALLOC_SOLS = 4096;
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_TAB = 0;
+constexpr size_t NA_N43 = align256(NA_TAB + sizeof(uint64_t) * NTAB * AP26_NSHIFT);
+constexpr size_t NA_SURV = align256(NA_N43 + sizeof(int64_t) * AP26_NN43);
+constexpr size_t NA_CNT = align256(NA_SURV + sizeof(int64_t) * NOALLOC_SURV);
+constexpr size_t NA_SOLS = align256(NA_CNT + sizeof(unsigned long long) * NCOUNT);
+constexpr size_t NA_BYTES = align256(NA_SOLS + sizeof(DevSol) * NOALLOC_SOLS);
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+/* Each device holds its own copy of `scratch` at its own address; the host
+   name is only a stand-in.  The address is looked up once per device (with
+   that device current) and kept here; host code uses it, never the name. */
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[MAX_DEVICES];
+
+static int scratch_address(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= MAX_DEVICES) return AP26_E_RANGE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return AP26_E_DEVICE;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return AP26_OK;
+}
+
 extern "C" {
 
 const char *ap26_strerror(int code) {
@@ -112,6 +157,7 @@ int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
       break;
     }
     ctx->surv_cap = SURV_CAP;
+    if ((rc = scratch_address(device, &ctx->scratch)) != AP26_OK) break;
     ctx->sieve = pick_variant(ctx->opts.sieve_variant);
     tune_load_cache(ctx);
     ctx->opts.block = ctx->sieve.block;
@@ -162,34 +208,50 @@ static double elapsed_ms(hipEvent_t a, hipEvent_t b) {
   return (double)ms;
 }
 
-/* Enqueue sieve of shifts [it0,it1) x n43 [j0,j1) into d_surv (counter C_NSURV).
-   Counters must have been cleared. */
-static int enqueue_sieve(ap26_ctx *ctx, const KParams &kp, int it0, int it1, int j0, int j1,
+/* Device buffers of one tile search: the context's own, or the
+   allocation-free entry point's carve of `scratch`. */
+struct SpanBufs {
+  uint64_t *tab;             /* NTAB x 10 words */
+  int64_t *n43;              /* [10840] */
+  int64_t *surv;
+  int64_t surv_cap;
+  unsigned long long *cnt;   /* NCOUNT counters */
+  DevSol *sols;
+  int32_t sol_cap;
+};
+
+static SpanBufs ctx_bufs(ap26_ctx *ctx) {
+  return SpanBufs{ctx->d_tab, ctx->d_n43, ctx->d_surv, ctx->surv_cap, ctx->d_cnt, ctx->d_sols, SOL_CAP};
+}
+
+/* Enqueue sieve of shifts [it0,it1) x n43 [j0,j1) into bufs.surv (counter
+   C_NSURV).  Counters must have been cleared. */
+static int enqueue_sieve(ap26_ctx *ctx, const SpanBufs &bf, const KParams &kp, int it0, int it1, int j0, int j1,
                          ap26_stats *st) {
   const int S = ctx->sieve.S;
   const int nslot = it1 - it0;
   const int npass = (nslot + S - 1) / S;
   /* tables for every shift slot of every pass */
   dim3 tgrid((541 + 63) / 64, NPT, npass * S);
-  hipLaunchKernelGGL(tables_kernel, tgrid, dim3(64), 0, ctx->stream, kp, it0, it1, S, ctx->d_tab);
+  hipLaunchKernelGGL(tables_kernel, tgrid, dim3(64), 0, ctx->stream, kp, it0, it1, S, bf.tab);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(ctx->ev[1], ctx->stream));
   const int grid = ctx->cu_count * ctx->opts.blocks_per_cu;
   for (int p = 0; p < npass; p++) {
     SieveArgs a;
     a.kp = kp;
-    a.n43 = ctx->d_n43;
-    a.tab = ctx->d_tab + (size_t)p * NTAB * S;
+    a.n43 = bf.n43;
+    a.tab = bf.tab + (size_t)p * NTAB * S;
     a.shift0 = kp.SHIFT + 64 * (it0 + p * S);
     a.nslot = (it1 - (it0 + p * S)) < S ? (it1 - (it0 + p * S)) : S;
     a.blk0 = (int64_t)j0 * AP26_N53_PER_N43;
     a.blk1 = (int64_t)j1 * AP26_N53_PER_N43;
-    a.surv = ctx->d_surv;
-    a.nsurv = ctx->d_cnt + C_NSURV;
-    a.cap = ctx->surv_cap;
+    a.surv = bf.surv;
+    a.nsurv = bf.cnt + C_NSURV;
+    a.cap = bf.surv_cap;
     a.surv_job = nullptr;
     a.job = 0;
-    a.err = (unsigned int *)(ctx->d_cnt + C_ERR);
+    a.err = (unsigned int *)(bf.cnt + C_ERR);
     for (int v = 0; v < NVW; v++) a.vc[v] = vw_offset(a.shift0, VW_P[v]);
     hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
@@ -275,7 +337,7 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     rc = prepare_k(ctx, K, SHIFT, &kp);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, &st);
+    rc = enqueue_sieve(ctx, ctx_bufs(ctx), kp, it0, it1, j0, j1, &st);
     if (rc) return rc;
     WalkArgs w;
     w.surv = ctx->d_surv;
@@ -343,6 +405,106 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
   return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
 }
 
+static_assert(sizeof(DevSol) == sizeof(ap26_sol) && offsetof(DevSol, first) == offsetof(ap26_sol, first) &&
+                  offsetof(DevSol, range_id) == offsetof(ap26_sol, range_id) &&
+                  offsetof(DevSol, prp2) == offsetof(ap26_sol, prp2_mask) &&
+                  offsetof(DevSol, valid) == offsetof(ap26_sol, valid_mask),
+              "device reports are copied out as ap26_sol");
+
+int ap26_search_span_noalloc(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1, int32_t j0,
+                             int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out, ap26_stats *stats) {
+  if (!ctx || !n_out || (cap > 0 && !out)) return AP26_E_RANGE;
+  if (!span_ok(K, SHIFT, it0, it1, j0, j1, true)) return AP26_E_RANGE;
+  if (j1 - j0 > NOALLOC_MAX_N43) { /* more than the scratch's survivor slots are sized for */
+    if (ctx->opts.verbose)
+      fprintf(stderr, "ap26_search_span_noalloc: %d n43 starts, at most %d\n", j1 - j0, NOALLOC_MAX_N43);
+    return AP26_E_RANGE;
+  }
+  if (!ctx->scratch) return AP26_E_DEVICE;
+  auto t0 = std::chrono::steady_clock::now();
+  HIPCHK(hipSetDevice(ctx->device));
+  ap26_stats st{};
+  *n_out = 0;
+  if (it0 == it1 || j0 == j1) {
+    if (stats) *stats = st;
+    return AP26_OK;
+  }
+  unsigned char *base = (unsigned char *)ctx->scratch;
+  const SpanBufs bf{(uint64_t *)(base + NA_TAB), (int64_t *)(base + NA_N43), (int64_t *)(base + NA_SURV),
+                    NOALLOC_SURV, (unsigned long long *)(base + NA_CNT), (DevSol *)(base + NA_SOLS),
+                    NOALLOC_SOLS};
+  KParams kp;
+  HostK hk;
+  host_kconsts(K, &hk);
+  kp.STEP = hk.STEP;
+  kp.n0 = hk.n0;
+  kp.S31 = hk.S[0];
+  kp.S37 = hk.S[1];
+  kp.S41 = hk.S[2];
+  kp.S43 = hk.S[3];
+  kp.S47 = hk.S[4];
+  kp.S53 = hk.S[5];
+  kp.S59 = hk.S[6];
+  kp.K = K;
+  kp.SHIFT = SHIFT;
+  for (int v = 0; v < NVW; v++) host_vwin(K, VW_P[v], kp.vw[v]);
+  /* the array keeps the last call's contents: clear the counters (a kernel,
+     no memset node), compute this K's n43 starts of the span on the device */
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, ctx->stream, bf.cnt, NCOUNT);
+  hipLaunchKernelGGL(n43_span_kernel, dim3((j1 - j0 + 63) / 64), dim3(64), 0, ctx->stream, kp, j0, j1, bf.n43);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
+  int rc = enqueue_sieve(ctx, bf, kp, it0, it1, j0, j1, &st);
+  if (rc) return rc;
+  WalkArgs w;
+  w.surv = bf.surv;
+  w.nsurv = bf.cnt + C_NSURV;
+  w.cap = bf.surv_cap;
+  w.STEP = kp.STEP;
+  w.K = K;
+  w.SHIFT = SHIFT;
+  w.range_id = -1;
+  w.sols = bf.sols;
+  w.nsol = (unsigned int *)(bf.cnt + C_NSOL);
+  w.sol_cap = bf.sol_cap;
+  w.ntests = bf.cnt + C_NTESTS;
+  w.err = (unsigned int *)(bf.cnt + C_ERR);
+  w.nlive = bf.cnt + C_LIVE;
+  w.big = bf.cnt + C_BIG;
+  hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+  hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, bf.sols, w.nsol, bf.sol_cap);
+  hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, bf.sols, w.nsol, bf.sol_cap);
+  HIPCHK(hipGetLastError());
+  st.walk_launches++;
+  HIPCHK(hipEventRecord(ctx->ev[3], ctx->stream));
+  unsigned long long cnt[NCOUNT];
+  HIPCHK(hipMemcpyAsync(cnt, bf.cnt, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(hipStreamSynchronize(ctx->stream));
+  if (cnt[C_ERR]) return AP26_E_INTERNAL;                                 /* a queue step on part of a wave */
+  if (cnt[C_NSURV] > (unsigned long long)bf.surv_cap) return AP26_E_INTERNAL; /* exceeds the sizing bound */
+  const unsigned int nsol = (unsigned int)(cnt[C_NSOL] & 0xffffffffu);
+  if (nsol > (unsigned int)bf.sol_cap) return AP26_E_INTERNAL;
+  st.k_searched = 1;
+  st.survivors = (int64_t)cnt[C_LIVE];
+  st.prp_tests = (int64_t)cnt[C_NTESTS];
+  st.reports = nsol;
+  st.ms_tables = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+  st.ms_sieve = elapsed_ms(ctx->ev[1], ctx->ev[2]);
+  st.ms_walk = elapsed_ms(ctx->ev[2], ctx->ev[3]);
+  if (cnt[C_BIG]) { /* the walk reached 2^62, where the reference's PrimeQ exits */
+    st.walk_past_limit = (int64_t)cnt[C_BIG];
+    if (stats) *stats = st;
+    return AP26_E_RANGE;
+  }
+  const int ncopy = (int)nsol < cap ? (int)nsol : cap;
+  if (ncopy > 0)
+    HIPCHK(hipMemcpy(out, bf.sols, sizeof(ap26_sol) * (size_t)ncopy, hipMemcpyDeviceToHost));
+  *n_out = (int32_t)nsol;
+  st.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  return (int)nsol > cap ? AP26_E_OVERFLOW : AP26_OK;
+}
+
 int ap26_search_k(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t iter_start, ap26_sol *out, int32_t cap,
                   int32_t *n_out, ap26_stats *stats) {
   return ap26_search_span(ctx, K, SHIFT, iter_start, AP26_NSHIFT, 0, AP26_NN43, out, cap, n_out, stats);
@@ -363,7 +525,7 @@ int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_
     rc = prepare_k(ctx, K, SHIFT, &kp);
     if (rc) return rc;
     HIPCHK(hipEventRecord(ctx->ev[0], ctx->stream));
-    rc = enqueue_sieve(ctx, kp, it0, it1, j0, j1, nullptr);
+    rc = enqueue_sieve(ctx, ctx_bufs(ctx), kp, it0, it1, j0, j1, nullptr);
     if (rc) return rc;
     unsigned long long err = 0;
     HIPCHK(hipMemcpyAsync(&nsurv, ctx->d_cnt + C_NSURV, sizeof(nsurv), hipMemcpyDeviceToHost, ctx->stream));

@@ -626,6 +626,47 @@ static __global__ void n43_kernel(const BatchJobDev *__restrict__ jobs, const Ro
   n43[(size_t)jl * AP26_NN43 + j] = v;
 }
 
+/* The guarded (i31, i37, i41) triples of the n43 enumeration in the
+   reference's order (src/opencl/AP26.h:125-136): n43 index j = 8*t + 4*i3
+   + i5 for triple t.  Built at compile time for kernels that compute the n43
+   starts of one K without a host table. */
+struct TripleTable {
+  uint32_t t[AP26_NN43 / 8];
+  int n;
+};
+
+constexpr TripleTable make_triples() {
+  TripleTable T{};
+  for (int i31 = 0; i31 < 7; i31++)
+    for (int i37 = 0; i37 < 13; i37++) {
+      if (i37 - i31 > 10 || i31 - i37 > 4) continue;
+      for (int i41 = 0; i41 < 17; i41++) {
+        if (i41 - i31 > 14 || i41 - i37 > 14 || i31 - i41 > 4 || i37 - i41 > 10) continue;
+        if (T.n < AP26_NN43 / 8) T.t[T.n] = (uint32_t)i31 | ((uint32_t)i37 << 8) | ((uint32_t)i41 << 16);
+        T.n++;
+      }
+    }
+  return T;
+}
+
+constexpr TripleTable TRIPLES = make_triples();
+static_assert(TRIPLES.n * 8 == AP26_NN43, "n43 starts per K");
+
+/* n43 starts [j0, j1) of one K into n43[j] (as n43_kernel does per job) */
+static __global__ void n43_span_kernel(KParams kp, int32_t j0, int32_t j1, int64_t *__restrict__ n43) {
+  const int j = j0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (j >= j1) return;
+  const uint32_t t = TRIPLES.t[j >> 3];
+  const int i31 = t & 0xff, i37 = (t >> 8) & 0xff, i41 = (t >> 16) & 0xff;
+  const int i3 = (j >> 2) & 1, i5 = j & 3;
+  int64_t v = kp.n0 + i3 * AP26_S3 + i5 * AP26_S5 + i31 * kp.S31 + i37 * kp.S37 + i41 * kp.S41;
+  int64_t q = (int64_t)((double)v * (1.0 / (double)AP26_MOD));
+  v -= q * AP26_MOD;
+  if (v < 0) v += AP26_MOD;
+  if (v >= AP26_MOD) v -= AP26_MOD;
+  n43[j] = v;
+}
+
 /* Tables of every (job, shift) of the round; grid z = job*10 + shift.  S = 1:
    [job][shift][entry]; S = 2: [job][shift pair][entry][shift in pair].  Shifts
    before the job's resume point get all-zero masks. */

@@ -77,6 +77,8 @@ SIGNATURES = {
                                      ctypes.POINTER(Stats)]),
     "ap26_search_span": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32,
                                         ctypes.POINTER(_I32), ctypes.POINTER(Stats)]),
+    "ap26_search_span_noalloc": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I32,
+                                                ctypes.POINTER(_I32), ctypes.POINTER(Stats)]),
     "ap26_sieve_span": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _P, _I64,
                                        ctypes.POINTER(_I64)]),
     "ap26_okok_table": (ctypes.c_int, [_P, _I32, _I32, _P]),
@@ -202,6 +204,17 @@ class Context:
         rc = self._lib.ap26_search_span(self.h, K, shift, it0, it1, j0, j1, _ptr(out), cap,
                                         ctypes.byref(n), ctypes.byref(st))
         _check(rc, "ap26_search_span", st)
+        return out[:n.value].copy(), st
+
+    def search_span_noalloc(self, K, shift, it0, it1, j0, j1, cap=1 << 12):
+        """ap26_search_span_noalloc: the tile search without any allocation
+        (at most 64 n43 starts)."""
+        out = np.zeros(cap, dtype=SOL_DTYPE)
+        n = ctypes.c_int32()
+        st = Stats()
+        rc = self._lib.ap26_search_span_noalloc(self.h, K, shift, it0, it1, j0, j1, _ptr(out), cap,
+                                                ctypes.byref(n), ctypes.byref(st))
+        _check(rc, "ap26_search_span_noalloc", st)
         return out[:n.value].copy(), st
 
     def sieve_span(self, K, shift, it0, it1, j0, j1, cap=1 << 22):

@@ -113,6 +113,23 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
                      int32_t j0, int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out,
                      ap26_stats *stats);
 
+/* ap26_search_span for callers that cannot allocate (code under HIP graph
+   capture, tight serving loops, harnesses that forbid hipMalloc): no
+   counterpart in the reference.  It calls no hipMalloc, hipFree or other
+   allocating routine, host or device: its device working memory is one
+   file-scope array of about 34 MB that every device the library runs on
+   holds until the process ends (tables of 10 shifts, n43 starts, 2^22
+   survivor slots, counters, 4096 reports), cleared by kernels as each call
+   needs.  The caller passes inputs and outputs as for ap26_search_span;
+   results, return codes and stats are the same.  Limit: at most 64 n43
+   starts per call (j1 - j0 <= 64), enough survivor slots for any K <= 6e8;
+   a larger span returns AP26_E_RANGE.  All calls on one GPU share that
+   GPU's array: calls that overlap on one GPU (from several threads or
+   contexts) race and must be serialised by the caller. */
+int ap26_search_span_noalloc(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1,
+                             int32_t j0, int32_t j1, ap26_sol *out, int32_t cap, int32_t *n_out,
+                             ap26_stats *stats);
+
 /* Sieve survivors only (the candidates the walk would test) of that tile,
    in no particular order.  *n_out = total; AP26_E_OVERFLOW if > cap. */
 int ap26_sieve_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32_t it1,
