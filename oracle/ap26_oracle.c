@@ -170,6 +170,37 @@ int ap26o_sprp(uint64_t b, uint64_t N) {
 
 int ap26o_prp2(uint64_t N) { return ap26o_sprp(2, N); }
 
+/* ap26o_prp2 over an array, split over threads (contiguous slices): the
+   checker of whole SPRP batches such as the 2^26-value micro-benchmark. */
+typedef struct {
+  const uint64_t *n;
+  uint8_t *out;
+  size_t b, e;
+} prp_slice;
+
+static void *prp_worker(void *arg) {
+  prp_slice *sl = (prp_slice *)arg;
+  for (size_t i = sl->b; i < sl->e; i++) sl->out[i] = (uint8_t)ap26o_prp2(sl->n[i]);
+  return NULL;
+}
+
+void ap26o_prp2_array(const uint64_t *n, uint8_t *out, size_t count, int32_t threads) {
+  if (threads < 1) threads = 1;
+  if ((size_t)threads > count) threads = count ? (int32_t)count : 1;
+  pthread_t *th = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+  prp_slice *sl = (prp_slice *)calloc((size_t)threads, sizeof(prp_slice));
+  for (int32_t t = 0; t < threads; t++) {
+    sl[t].n = n;
+    sl[t].out = out;
+    sl[t].b = count * (size_t)t / (size_t)threads;
+    sl[t].e = count * (size_t)(t + 1) / (size_t)threads;
+    pthread_create(&th[t], NULL, prp_worker, &sl[t]);
+  }
+  for (int32_t t = 0; t < threads; t++) pthread_join(th[t], NULL);
+  free(th);
+  free(sl);
+}
+
 /* src/cpu/cpusse2.cpp:570-590 (GPU state machine src/opencl/kernels/checkn.cl:160-202):
    count base-2 SPRPs forward from n+5*STEP; only if that count reaches 10 extend
    backward from n+4*STEP while the term is positive. */

@@ -63,6 +63,8 @@ int ap26o_sprp(uint64_t b, uint64_t N);
 /* Base-2 test the search uses (src/common/PrimeQ_x86.h:160-169, without the
    2^62 exit). */
 int ap26o_prp2(uint64_t N);
+/* ap26o_prp2 of count values on `threads` threads */
+void ap26o_prp2_array(const uint64_t *n, uint8_t *out, size_t count, int32_t threads);
 
 /* AP walk from sieve survivor n (src/cpu/cpusse2.cpp:570-590): returns k and
    writes the first term; k < 10 means nothing is reported. */

@@ -38,6 +38,7 @@ def load():
             "ap26o_okok_table": (None, [I32, I32, P]),
             "ap26o_sprp": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64]),
             "ap26o_prp2": (ctypes.c_int, [ctypes.c_uint64]),
+            "ap26o_prp2_array": (None, [P, P, ctypes.c_size_t, I32]),
             "ap26o_walk": (I32, [I64, I64, ctypes.POINTER(I64)]),
             "ap26o_sieve_span": (I64, [I32, I32, I32, I32, I32, I32, P, I64]),
             "ap26o_search_span": (I64, [I32, I32, I32, I32, I32, I32, P, I64,
@@ -114,9 +115,11 @@ def prp2(n):
     return bool(load().ap26o_prp2(n))
 
 
-def prp2_array(arr):
-    lib = load()
-    return np.fromiter((lib.ap26o_prp2(int(x)) for x in arr), dtype=np.uint8, count=len(arr))
+def prp2_array(arr, threads=1):
+    arr = np.ascontiguousarray(arr, dtype=np.uint64)
+    out = np.zeros(arr.size, dtype=np.uint8)
+    load().ap26o_prp2_array(_ptr(arr), _ptr(out), ctypes.c_size_t(arr.size), threads)
+    return out
 
 
 def walk(n, step):

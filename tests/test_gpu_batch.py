@@ -8,6 +8,7 @@ import pytest
 
 from ap26_amd import gpu
 from ap26_amd import host as H
+from ap26_amd import workloads as W
 from conftest import GOLDEN, read_sol
 
 pytestmark = pytest.mark.gpu
@@ -294,3 +295,51 @@ def test_batch_equals_plain_on_mixed_workload_sample(gpu_ctx):
         assert key([x for x in sols if x["range_id"] == i]) == sorted(want), units[i]
         assert tot[i] == sum(f % 1000 + k for k, _, f in want), units[i]
     assert st.survivors == plain_surv and st.k_searched == got
+
+
+def _expected(golden_index, K, shift):
+    g = [x for x in golden_index if x["kmin"] == x["kmax"] == K and x["shift"] == shift]
+    assert g, f"no reference fixture for K={K} SHIFT {shift}"
+    return read_sol(os.path.join(GOLDEN, "ref_min10", g[0]["name"] + ".sol"))
+
+
+def _unit_sol(tmp, i, unit, sols):
+    d = tmp / str(i)
+    d.mkdir()
+    h = H.Host(unit[0], unit[1], unit[2], str(d), min_report=10)
+    for s in sorted(sols.tolist(), key=lambda s: (s[2], s[0])):
+        assert h.report(s[0], s[1], s[2], s[5], s[6]) == 0
+    assert h.write_hash() == 0
+    h.close()
+    return read_sol(str(d / "SOL-AP26.txt"))
+
+
+def test_bench_units_against_reference(golden_index, gpu_ctx, tmp_path):
+    """Every K the benchmark times -- the batched leg's sample of the mixed
+    workload (48 searched K over units of all three SHIFTs, across rounds)
+    and the plain leg's first 24 searched K at SHIFT 640 -- plus every 256th
+    searched K of the whole 4096-K plain-path unit, searched as units of one
+    batched call: each unit's SOL records and hash line equal the reference
+    build's (a multi-K unit's hash composed from its K's totals)."""
+    units, _, ks = W.c4_sample(48)
+    units = list(units) + [(K, K, 640, 0) for K in W.c3_ks(24)] + [(K, K, 640, 0) for K in W.c3_stratified(256)]
+    ks = list(ks) + [[K] for K in W.c3_ks(24)] + [[K] for K in W.c3_stratified(256)]
+    sols, tot, st = gpu_ctx.search_batch(units)
+    assert st.k_searched == sum(len(k) for k in ks)
+    for i, (u, kk) in enumerate(zip(units, ks)):
+        recs, T = [], 0
+        for K in kk:
+            r, hl = _expected(golden_index, K, u[2])
+            recs += r
+            T += int(hl[8:], 16)
+        want = (sorted(recs), H.hash_line(u[0], u[1], H.cap_total(T)))
+        assert _unit_sol(tmp_path, i, u, sols[sols["range_id"] == i]) == want, u
+
+
+def test_stratified_plain_path_against_reference(golden_index, gpu_ctx, tmp_path):
+    """Every 256th searched K of the 4096-K plain-path unit (SHIFT 640) on
+    the plain per-K path: SOL records and hash line equal the reference
+    build's single-K runs."""
+    for i, K in enumerate(W.c3_stratified(256)):
+        sols, _ = gpu_ctx.search_k(K, 640)
+        assert _unit_sol(tmp_path, i, (K, K, 640), sols) == _expected(golden_index, K, 640), K

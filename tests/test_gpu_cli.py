@@ -235,3 +235,15 @@ def test_walk_limit_exits_like_the_reference(tmp_path):
         assert fa.exists() == fb.exists()
         if fa.exists():
             assert fa.read_text().split()[:6] == fb.read_text().split()[:6]
+
+
+def test_config1_whole_unit(golden_index, tmp_path):
+    """The CPU-runnable configuration as one unit: `AP26 875858 875873 640
+    -report 10` (16 consecutive K, 13 searched) writes the records of the
+    reference build's 13 single-K runs and the unit hash composed from their
+    totals."""
+    r = run([875858, 875873, 640, "-report", 10], tmp_path)
+    assert r.returncode == 0, r.stderr
+    names = [f"c1_{K}" for K in range(875858, 875874) if H.will_search(K)]
+    assert len(names) == 13
+    assert read_sol(str(tmp_path / "SOL-AP26.txt")) == _units_hash(golden_index, names, 875858, 875873)

@@ -36,13 +36,14 @@ def test_prp_edges(gpu_ctx):
     assert gpu_ctx.prp_batch(n).tolist() == [int(O.prp2(v)) for v in vals]
 
 
-def test_prp_microbench_distribution_sample(gpu_ctx):
-    """2^20 odd values uniform in [2^58, 2^63), the PRP micro-benchmark's input
-    distribution (the benchmark itself runs 2^26)."""
+def test_prp_microbench_full_against_oracle(gpu_ctx):
+    """All 2^26 inputs of the PRP micro-benchmark (odd values uniform in
+    [2^58, 2^63), numpy default_rng(26), as bench.py draws them): the device's
+    booleans equal the oracle's SPRP on every one (threaded oracle)."""
     rng = np.random.default_rng(26)
-    n = rng.integers(2 ** 58, 2 ** 63, size=1 << 20, dtype=np.uint64) | np.uint64(1)
+    n = rng.integers(2 ** 58, 2 ** 63, size=1 << 26, dtype=np.uint64) | np.uint64(1)
     g = gpu_ctx.prp_batch(n)
-    o = O.prp2_array(n)
+    o = O.prp2_array(n, threads=max(1, min(16, len(os.sched_getaffinity(0)))))
     assert np.array_equal(g, o) and 0 < o.sum() < len(o)
 
 
@@ -103,20 +104,24 @@ def test_full_k_against_reference_golden(gpu_ctx, golden_index, tmp_path):
         assert got == want, g["name"]
 
 
-def _host_masks(k, K, first):
+def _oracle_masks(k, K, first):
+    """Per-term bits from the oracle: bit j of the first word when term j is a
+    base-2 strong PRP (ap26o_sprp), of the second when it is one to all nine
+    bases 2..23 (ap26o_validate on the one-term progression, the reference's
+    validate_ap26 rule, AP26.cpp:513-543)."""
     p2 = v9 = 0
     for j in range(min(k, 32)):
         n = first + j * K * PRIM23
-        if n >= 3 and H.sprp(2, n):
+        if n >= 3 and O.sprp(2, n):
             p2 |= 1 << j
-            if all(H.sprp(b, n) for b in (3, 5, 7, 11, 13, 17, 19, 23)):
+            if O.validate(1, K, n, 9) == 1:
                 v9 |= 1 << j
     return p2, v9
 
 
 def test_device_verification_masks(gpu_ctx):
     """9-base verification on the device (validate_ap26 / val_base2_ap26,
-    AP26.cpp:484-543) against the host's strong_prp restatement: every
+    AP26.cpp:484-543) against the oracle's SPRP and validation: every
     report of two K, the reference's report vectors (APs holding base-2
     strong pseudoprimes), and APs built around A001262 pseudoprimes."""
     import json
@@ -126,7 +131,7 @@ def test_device_verification_masks(gpu_ctx):
     rows += [(int(s["k"]), int(s["K"]), int(s["first"]), int(s["prp2_mask"]), int(s["valid_mask"])) for s in sols2]
     assert len(rows) > 300
     for k, K, f, p2, v9 in rows:
-        assert (p2, v9) == _host_masks(k, K, f), (k, K, f)
+        assert (p2, v9) == _oracle_masks(k, K, f), (k, K, f)
     vecs = json.load(open(os.path.join(GOLDEN, "report_vectors.json")))
     extra = [tuple(r) for v in vecs for r in v["reports"]]
     # pseudoprimes to base 2 that fail other bases, as first and middle terms
@@ -137,7 +142,7 @@ def test_device_verification_masks(gpu_ctx):
         arr[i]["k"], arr[i]["K"], arr[i]["first"] = k, K, f
     out = gpu_ctx.verify_sols(arr)
     for i, (k, K, f) in enumerate(extra):
-        assert (int(out[i]["prp2_mask"]), int(out[i]["valid_mask"])) == _host_masks(k, K, f), (k, K, f)
+        assert (int(out[i]["prp2_mask"]), int(out[i]["valid_mask"])) == _oracle_masks(k, K, f), (k, K, f)
 
 
 def test_survivor_count_of_reference_comment(gpu_ctx):

@@ -258,3 +258,61 @@ def test_oracle_full_k_against_reference_golden():
     want, hline = read_sol(os.path.join(GOLDEN, "ref_min10", "t366384.sol"))
     assert sorted(recs) == want
     assert "%016X" % O.hash_word(366384, 366384, h) == hline
+
+
+# The reference GPU kernel's early-exit grouping of the 83 sieve primes
+# (src/opencl/kernels/sieve.cl:32-114): a mask is ANDed over a group and
+# tested once per group; groups of 5, 5, 5, 4, 4 primes, then pairs.
+REFERENCE_GROUPS = [5, 5, 5, 4, 4] + [2] * 30
+
+
+@pytest.mark.parametrize("K,shift,it,j", [(875859, 640, 3, 123), (366384, 0, 9, 10000)])
+def test_lookup_count_against_independent_count(K, shift, it, j):
+    """The sieve's work count behind its roofline (ap26o_lookup_count)
+    equals a count written here with numpy: every position of one n43 start
+    (n59 = n43 + i43*S43 + i47*S47 + i53*S53 + i59*S59 mod MOD, 443,555 of
+    them) is ANDed group by group over the oracle's OKOK table (pinned by
+    test_okok_table_definition) and charged each group it enters."""
+    assert sum(REFERENCE_GROUPS) == len(SIEVE_PRIMES) == 83
+    kc = O.kconsts(K)
+    n43 = int(O.n43_table(K)[j])
+    i43, i47, i53, i59 = np.meshgrid(np.arange(19), np.arange(23), np.arange(29), np.arange(35), indexing="ij")
+    n59 = (n43 + i43 * kc.S43 + i47 * kc.S47 + i53 * kc.S53 + i59 * kc.S59).ravel() % MOD
+    tab = O.okok_table(K, shift + 64 * it)
+    off = np.cumsum([0] + SIEVE_PRIMES[:-1])
+    m = np.full(n59.size, np.uint64(2 ** 64 - 1), dtype=np.uint64)
+    alive = np.ones(n59.size, dtype=bool)
+    count, i = 0, 0
+    for g in REFERENCE_GROUPS:
+        count += g * int(alive.sum())
+        for p, o in zip(SIEVE_PRIMES[i:i + g], off[i:i + g]):
+            m &= tab[o + n59 % p]
+        alive &= m != 0
+        i += g
+    assert count == O.lookup_count(K, shift, it, it + 1, j, j + 1)
+    assert 18 * n59.size < count < 21 * n59.size  # ~18-19.5 lookups per mask
+
+
+def test_oracle_n43_starts_reproduce_reference_records(golden_index):
+    """The oracle's n43 values and order, without a full-K run: for two K of
+    the golden set, the n43 starts that carry reports (index j of the
+    enumeration, tests/golden/n43_report_index.json) are searched on the CPU
+    over all 10 shifts; at the first, a middle and the last of them, and at
+    a start of no report, the oracle's reports are exactly the index's, and
+    every one is a record of the reference build's SOL file (k >= 10)."""
+    with open(os.path.join(GOLDEN, "n43_report_index.json")) as f:
+        index = json.load(f)
+    assert len(index) == 2
+    for u in index:
+        recs, _ = read_sol(os.path.join(GOLDEN, "ref_min10", u["golden"] + ".sol"))
+        rec_set = {(k, first) for k, _, first in recs}
+        starts = u["starts"]
+        idx_pairs = {(k, f) for s in starts for k, f in s["reports"]}
+        assert {p for p in idx_pairs if p[0] >= 10} == rec_set  # the index covers every record
+        with_reports = {s["j"] for s in starts}
+        empty = next(j for j in range(10840) if j not in with_reports)
+        for s in (starts[0], starts[len(starts) // 2], starts[-1], {"j": empty, "reports": []}):
+            sols, _ = O.search_span(u["K"], u["shift"], 0, 10, s["j"], s["j"] + 1, threads=4)
+            got = sorted((int(x["k"]), int(x["first"])) for x in sols)
+            assert got == sorted(tuple(r) for r in s["reports"]), (u["K"], s["j"])
+            assert all((k, f) in rec_set for k, f in got if k >= 10)
