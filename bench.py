@@ -39,10 +39,11 @@ Beside it on the line:
                 every unit's result compared with the batched run's (alone vs
                 in the mix).  Unit length floor(exp(U(0, ln 16))): the
                 workload's 1..8192 scaled down so a sweep finishes in the bench;
-  cpu_baseline  the oracle (a from-scratch C restatement of the reference CPU
-                search) on a bounded sample of the plain workload on all the
-                host cores the box grants, plus its SPRP on a sample of the
-                micro-benchmark inputs, checked against the GPU's outputs.
+  cpu_baseline  the reference CPU build's own AVX2 search, compiled from its
+                sources (oracle/_ref), one process per host core the box
+                grants, each on ~20 s of a K of the plain workload; plus the
+                oracle's SPRP on a sample of the micro-benchmark inputs,
+                checked against the GPU's outputs.
 """
 import argparse
 import json
@@ -370,28 +371,70 @@ def cpu_info():
     return model, avail
 
 
+REF_SAMPLE = os.path.join(ROOT, "oracle", "_ref", "ref_cpu_sample")
+
+
+def reference_cpu_sample(procs, seconds=20.0):
+    """The reference CPU build's own AVX2 search (Search_avx2, compiled from
+    the reference's sources into oracle/_ref) on a bounded sample: one
+    process per core, each on a different K of the plain workload at SHIFT
+    640, stopped after `seconds` at a boundary of its n43 blocks (each
+    4/850 of a K in the first group of 4 shifts; oracle/ref_cpu_sample.cpp).
+    Returns per-process K/s."""
+    import subprocess
+    tmp = tempfile.mkdtemp(prefix="ap26cpu")
+    ks = W.c3_ks(procs)
+    runs = []
+    for i, K in enumerate(ks):
+        d = os.path.join(tmp, str(i))
+        os.makedirs(d)
+        runs.append(subprocess.Popen([REF_SAMPLE, str(K), str(W.C3_SHIFT), str(seconds)], cwd=d,
+                                     stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True))
+    rates = []
+    for p in runs:
+        out, _ = p.communicate(timeout=seconds * 4 + 120)
+        f = out.split()
+        blocks, secs = int(f[f.index("blocks") + 1]), float(f[f.index("seconds") + 1])
+        if p.returncode != 0 or blocks < 1 or secs <= 0:
+            raise RuntimeError("reference CPU sample failed")
+        rates.append(blocks * 4 / 850 / secs)
+    return ks, rates
+
+
 def cpu_baseline(n_prp, gpu_prp, j1=2048):
-    """The oracle on a bounded sample: a slice of one K of the plain workload
-    (all 10 shifts, n43 starts 0..j1-1), one thread per core, and its SPRP on
-    2^16 micro-benchmark inputs, checked against the GPU's outputs."""
+    """The CPU baseline on the GPU box's host cores: the reference CPU build
+    itself (kind "reference") when oracle/_ref holds its sampler, else the
+    oracle's scalar restatement (kind "port") on a slice of one K; plus the
+    oracle's SPRP on 2^16 micro-benchmark inputs, checked against the GPU's
+    outputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     model, avail = cpu_info()
     threads = max(1, min(16, avail))
-    t = time.perf_counter()
-    O.search_span(W.C3_K0, W.C3_SHIFT, 0, 10, 0, j1, threads=threads)
-    dt = time.perf_counter() - t
-    frac = j1 / 10840
     s = n_prp[: 1 << 16]
     t = time.perf_counter()
     want = O.prp2_array(s)
     dtp = time.perf_counter() - t
+    prp = {"tests_per_s_1thread": s.size / dtp, "sample": int(s.size),
+           "mismatches_vs_gpu": int((want != gpu_prp[: s.size]).sum()) if gpu_prp is not None else None}
+    if os.access(REF_SAMPLE, os.X_OK):
+        ks, rates = reference_cpu_sample(threads)
+        kps = float(sum(rates))
+        return {"value": kps, "unit": "K/s", "cores": threads, "kind": "reference",
+                "cpu_model": model, "cores_available": avail, "s_per_k_per_core": float(np.mean([1 / r for r in rates])),
+                "sample": f"the reference CPU build's AVX2 search (Search_avx2, built from its sources), "
+                          f"{threads} processes, one per core, each ~20 s of one K (K={ks[0]}..{ks[-1]}, "
+                          f"SHIFT {W.C3_SHIFT}, shifts 0..3) timed at its n43-block progress points",
+                "prp": prp}
+    t = time.perf_counter()
+    O.search_span(W.C3_K0, W.C3_SHIFT, 0, 10, 0, j1, threads=threads)
+    dt = time.perf_counter() - t
+    frac = j1 / 10840
     return {"value": frac / dt, "unit": "K/s", "cores": threads, "kind": "port",
             "cpu_model": model, "cores_available": avail, "s_per_k_per_core": dt / frac * threads,
             "sample": f"oracle search of K={W.C3_K0} SHIFT {W.C3_SHIFT}, all 10 shifts, n43 starts "
                       f"0..{j1 - 1} ({frac * 100:.1f}% of one K) in {dt:.1f} s on {threads} threads",
-            "prp": {"tests_per_s_1thread": s.size / dtp, "sample": int(s.size),
-                    "mismatches_vs_gpu": int((want != gpu_prp[: s.size]).sum()) if gpu_prp is not None else None}}
+            "prp": prp}
 
 
 def progress(msg):
