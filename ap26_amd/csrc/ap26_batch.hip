@@ -45,9 +45,9 @@ PersistVariant make_pvariant2() {
 }
 
 /* shift pairs with the bank-rotated level-0 group (primes 61..137) */
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, class TU = Tune<>>
 PersistVariant make_pvariant2r() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N>};
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
 }
 
 /* 1..3, 9: one shift per item; 10..40, 50..56: shift pairs.  Level 0
@@ -55,10 +55,12 @@ PersistVariant make_pvariant2r() {
    16..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
    so level 0 pays off further than on one shift (61..131).  35..38: the first
    1..3 level-0 primes from register bit windows; 39..40: the last 2 or 4
-   level-0 primes read through L1 instead of LDS.  50..56 (0 = 54, the
+   level-0 primes read through L1 instead of LDS.  50..74 (0 = 70, the
    default): level 0 opens with the bank-rotated group of 61..137
    (level0_rot), whose 64 VGPRs of per-lane constants leave room for 12 waves
-   per CU (16 waves spill in the hot loop, 8 lose the latency hiding). */
+   per CU (16 waves spill in the hot loop, 8 lose the latency hiding); 57..74
+   vary the level boundaries, the wave priorities and how many rotated
+   gathers are in flight at once (Tune). */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -102,8 +104,28 @@ PersistVariant pick_pvariant(int id) {
     case 54: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>>();
     case 55: return make_pvariant2r<29, 12, Levels<0, 20, 29, 51, 83>>();
     case 56: return make_pvariant2r<36, 8, Levels<0, 18, 36, 55, 83>>();
+    /* 57..66: variant 54 with other wave priorities (level 0, level-1
+       drains, deeper drains) and level-0 splits */
+    case 57: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<0, 0, 0, 2>>();
+    case 58: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 3, 1, 1>>();
+    case 59: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 3, 1, 4>>();
+    case 60: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<1, 2, 0, 2>>();
+    case 61: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<3, 2, 1, 2>>();
+    case 62: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 2, 2, 2>>();
+    case 63: return make_pvariant2r<30, 12, Levels<0, 18, 30, 55, 83>>();
+    case 64: return make_pvariant2r<30, 12, Levels<0, 18, 30, 45, 83>>();
+    case 65: return make_pvariant2r<30, 12, Levels<0, 17, 30, 51, 83>>();
+    case 66: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>>();
+    case 67: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<2, 3, 1, 1>>();
+    case 68: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<3, 2, 1, 1>>();
+    case 69: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<3, 2, 1, 1>>();
+    case 70: return make_pvariant2r<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>();
+    case 71: return make_pvariant2r<30, 12, Levels<0, 19, 30, 53, 83>, Tune<2, 3, 1, 1>>();
+    case 72: return make_pvariant2r<30, 12, Levels<0, 19, 30, 49, 83>, Tune<2, 3, 1, 1>>();
+    case 73: return make_pvariant2r<29, 12, Levels<0, 19, 29, 51, 83>, Tune<2, 3, 1, 1>>();
+    case 74: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<3, 3, 1, 1>>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
-    default: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>>(); /* = 54 */
+    default: return make_pvariant2r<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>(); /* = 70 */
   }
 }
 
@@ -1017,7 +1039,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {54, 53, 55, 52, 28, 32, 22, 10, 9};
+static const int TUNE_VARIANTS[] = {70, 67, 58, 54, 28, 53, 55, 32, 10};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -52,9 +52,9 @@ SieveVariant make_q2variant() {
 }
 
 /* shift pairs with the bank-rotated level-0 group (primes 61..137) */
-template <int NL, int WAVES, class LV>
+template <int NL, int WAVES, class LV, class TU = Tune<>>
 SieveVariant make_q2variantr() {
-  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_N>};
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
 }
 
 

@@ -20,13 +20,13 @@ using namespace ap26;
 
 #include "ap26_ctx.h"
 
-/* Sieve variants: 0 = default (42: shift-pair queued sieve with the
-   bank-rotated level-0 group, level 0 on 18 primes, 12 waves per CU), 1 =
+/* Sieve variants: 0 = default (45: shift-pair queued sieve with the
+   bank-rotated level-0 group, level 0 on 20 primes, 12 waves per CU), 1 =
    simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve tuning
    candidates (LDS primes, waves per CU, levels, VGPR constants), 31..33 =
    shift-pair queued sieves without the rotation group, 34..36 = shift-pair
    with the first 1..3 level-0 primes from register bit windows instead of
-   LDS, 40..44 = rotation-group forms. */
+   LDS, 40..45 = rotation-group forms. */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -63,7 +63,8 @@ SieveVariant pick_variant(int id) {
     case 42: return make_q2variantr<30, 12, Levels<0, 18, 30, 51, 83>>();
     case 43: return make_q2variantr<36, 8, Levels<0, 18, 36, 55, 83>>();
     case 44: return make_q2variantr<21, 16, Levels<0, 16, 21, 51, 83>>();
-    default: return make_q2variantr<30, 12, Levels<0, 18, 30, 51, 83>>(); /* = 42 */
+    case 45: return make_q2variantr<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>();
+    default: return make_q2variantr<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>(); /* = 45 */
   }
 }
 

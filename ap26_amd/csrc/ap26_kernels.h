@@ -838,27 +838,29 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
 /* Level 0 of a fresh position with the rotation group: both masks of the
    pair over primes 0..E-1, the first ROT_N through the rotated rows, the rest
    (ROT_N..E-1) from the packed LDS tables. */
-template <int E>
+template <int E, int SPLIT = 2>
 __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConsts &rc, uint64_t &m0,
                                            uint64_t &m1) {
+  static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
+  constexpr int PART = ROT_N / SPLIT;
   static_assert(E >= ROT_N, "level 0 holds the rotation group");
   const double x = xd - TWO52;
   const uint32_t xs256 = lo32(xd) * 256u;
   const uint32_t xs16 = lo32(xd) * 16u;
   uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
-  /* two halves of the group, so that at most half its gathers are in flight */
-  static_for<0, 2>([&](auto HC) {
+  /* SPLIT parts of the group, so that at most 16/SPLIT gathers are in flight */
+  static_for<0, SPLIT>([&](auto HC) {
     constexpr int H = decltype(HC)::value;
-    uint4 w[ROT_N / 2];
-    static_for<0, ROT_N / 2>([&](auto KC) {
-      constexpr int k = H * (ROT_N / 2) + decltype(KC)::value;
+    uint4 w[PART];
+    static_for<0, PART>([&](auto KC) {
+      constexpr int k = H * PART + decltype(KC)::value;
       const double qd = __fma_rn(x, rc.inv[k], SHIFTER);
       int r; /* 256 * (centred residue of prime (lane + k) mod 16) */
       asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"((int)lo32(qd)), "v"(rc.m[k]), "v"(xs256));
       w[decltype(KC)::value] = *(const uint4 *)(ldsb + 256 * ROT_HALF + r + rc.col[k]);
     });
-    interleave_residues_and_reads<ROT_N / 2>();
-    static_for<0, ROT_N / 2>([&](auto KC) {
+    interleave_residues_and_reads<PART>();
+    static_for<0, PART>([&](auto KC) {
       constexpr int k = decltype(KC)::value;
       a0 &= w[k].x;
       a1 &= w[k].y;
@@ -1032,6 +1034,14 @@ __device__ __forceinline__ void close_survivors2(const SurvCursor &cur, int64_t 
    equal priorities for the two phases 180, level 0 alone raised 181. */
 constexpr int PRIO_LEVEL0 = 2, PRIO_LEVEL1 = 3, PRIO_DEEP = 1;
 
+/* Tuning of the rotated shift-pair sieve: wave priorities of the level-0
+   step, the level-1 drains and the deeper drains, and the number of parts
+   the rotated level-0 gathers are issued in. */
+template <int P0 = PRIO_LEVEL0, int P1 = PRIO_LEVEL1, int PD = PRIO_DEEP, int SPLIT = 2>
+struct Tune {
+  static constexpr int prio0 = P0, prio1 = P1, prio_deep = PD, split = SPLIT;
+};
+
 /* Queue capacity per wave: queue 0 holds Q0 entries, the deeper ones 128.
    With Q0 = 192 both pushes of a position (one per shift of the pair) go in
    before one drain: a single call site of the drain chain per position
@@ -1041,7 +1051,7 @@ constexpr int qsieve2_wave_entries() {
   return Q0 + 128 * (LV::N - 2);
 }
 
-template <int NL, class LV, int R0 = 0, int Q0 = 128>
+template <int NL, class LV, int R0 = 0, int Q0 = 128, class TU = Tune<>>
 struct QSieve2 {
   static_assert(Q0 == 128 || Q0 == 192, "queue 0 holds one or two pushes past a full wave");
   static constexpr int NQ = LV::N - 1;
@@ -1091,7 +1101,7 @@ struct QSieve2 {
       qn[L] -= 64;
       const QEntry e = q(L)[qn[L] + lane];
       __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_s_setprio(L == 0 ? PRIO_LEVEL1 : PRIO_DEEP);
+      __builtin_amdgcn_s_setprio(L == 0 ? TU::prio1 : TU::prio_deep);
       run_level<L>(e, true);
       __builtin_amdgcn_s_setprio(0);
     }
@@ -1118,8 +1128,8 @@ struct QSieve2 {
   }
 };
 
-template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128>
-__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0> &qs, const SieveArgs &a, int64_t tid,
+template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>>
+__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride, const RotConsts *rc = nullptr) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
@@ -1133,9 +1143,9 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0> &qs, cons
     for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
       qs.check_full_wave();
       uint64_t m0, m1;
-      __builtin_amdgcn_s_setprio(PRIO_LEVEL0);
+      __builtin_amdgcn_s_setprio(TU::prio0);
       if constexpr (R0)
-        level0_rot<LV::B[1]>(xd, qs.ldsb, *rc, m0, m1);
+        level0_rot<LV::B[1], TU::split>(xd, qs.ldsb, *rc, m0, m1);
       else
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
       __builtin_amdgcn_s_setprio(0);
@@ -1167,7 +1177,7 @@ constexpr int qsieve2_lds_bytes() {
 /* Items (job, shift pair, n43 chunk); tables [entry][shift in pair].  V: the
    first V level-0 primes from registers (vwin_pair) instead of LDS.  R0 =
    ROT_N: level 0 starts with the bank-rotated group (level0_rot). */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_persistent2_kernel(PersistArgs p) {
   constexpr int Q0 = R0 ? 192 : 128;
@@ -1225,23 +1235,23 @@ sieve_persistent2_kernel(PersistArgs p) {
     a.surv_job = p.surv_job;
     a.err = p.err;
     a.job = jl;
-    QSieve2<NL, LV, R0, Q0> qs;
+    QSieve2<NL, LV, R0, Q0, TU> qs;
     qs.lane = threadIdx.x & 63;
     qs.qb = L.queues[threadIdx.x >> 6];
-    for (int i = 0; i < QSieve2<NL, LV, R0, Q0>::NQ; i++) qs.qn[i] = 0;
+    for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU>::NQ; i++) qs.qn[i] = 0;
     qs.partial = false;
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V, G, R0, Q0>(qs, a, threadIdx.x, blockDim.x, &rc);
+    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU>(qs, a, threadIdx.x, blockDim.x, &rc);
   }
 }
 
 /* One launch per (K, shift pair): the plain path with the shift-pair sieve
    (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
    both shifts), survivors in per-wave slot chunks as in the batched engine. */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_queue2_kernel(SieveArgs a) {
   constexpr int Q0 = R0 ? 192 : 128;
@@ -1256,17 +1266,17 @@ sieve_queue2_kernel(SieveArgs a) {
   RotConsts rc;
   if constexpr (R0) rc.init(threadIdx.x & 63);
   __syncthreads();
-  QSieve2<NL, LV, R0, Q0> qs;
+  QSieve2<NL, LV, R0, Q0, TU> qs;
   qs.lane = threadIdx.x & 63;
   qs.qb = L.queues[threadIdx.x >> 6];
-  for (int i = 0; i < QSieve2<NL, LV, R0, Q0>::NQ; i++) qs.qn[i] = 0;
+  for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU>::NQ; i++) qs.qn[i] = 0;
   qs.partial = false;
   qs.ldsb = (const char *)L.tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV, V, G, R0, Q0>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
-                                       (int64_t)gridDim.x * blockDim.x, &rc);
+  qsieve2_blocks<NL, LV, V, G, R0, Q0, TU>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                           (int64_t)gridDim.x * blockDim.x, &rc);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }
 
