@@ -90,16 +90,18 @@ PEAK_HBM_GBS = 8000.0
 # GPU, 1.256 wave-instructions per CU-cycle (two-source 32-bit integer
 # instructions at 32 waves per CU; three-source ones and multiplies reach
 # 0.82-0.92), so rate / ceiling is the issue-slot use.  The nominal INT32
-# peak above assumes 1.0.
-SIEVE_VALU_LANE_INSTR_PER_K = 4.6064e12
+# peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
+# nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
+# sieve's load, so per-clock rates are about 6 % higher).
+SIEVE_VALU_LANE_INSTR_PER_K = 4.4769e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 149,308 + 7,326,350) KiB / 16 K.
-TRAFFIC_BYTES_PER_K = 4.880e8
+# (profiles/sieve_rot_traffic.md): (2 x 197,328 + 11,496,800) KiB / 16 K.
+TRAFFIC_BYTES_PER_K = 7.611e8
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
 # n43 starts (10,840 x 8 B) read once.
@@ -374,31 +376,56 @@ def cpu_info():
 REF_SAMPLE = os.path.join(ROOT, "oracle", "_ref", "ref_cpu_sample")
 
 
+def n43_per_block():
+    """n43 starts in each of the 85 progress blocks of one shift group: the
+    reference's (i31, i37) blocks and, inside each, its i41 x i3 x i5 loops
+    with their guards (src/cpu/cpuavx2.cpp:377-393); they sum to 10,840."""
+    counts = []
+    for i31 in range(7):
+        for i37 in range(13):
+            if i37 - i31 <= 10 and i31 - i37 <= 4:
+                n41 = sum(1 for i41 in range(17)
+                          if i41 - i31 <= 14 and i41 - i37 <= 14 and i31 - i41 <= 4 and i37 - i41 <= 10)
+                counts.append(8 * n41)
+    return counts
+
+
 def reference_cpu_sample(procs, seconds=20.0):
     """The reference CPU build's own AVX2 search (Search_avx2, compiled from
     the reference's sources into oracle/_ref) on a bounded sample: one
     process per core, each on a different K of the plain workload at SHIFT
-    640, stopped after `seconds` at a boundary of its n43 blocks (each
-    4/850 of a K in the first group of 4 shifts; oracle/ref_cpu_sample.cpp).
-    Returns per-process K/s."""
+    640, stopped after `seconds` at a boundary of its n43 blocks
+    (oracle/ref_cpu_sample.cpp).  The search runs the K's ten shifts as three
+    groups of four vector lanes (shifts 0-3, 4-7, and 8-9 with two lanes
+    masked), so half of the processes time the first group and half the
+    last; the blocks done are counted in n43 starts (n43_per_block), and one
+    K costs two first-group passes plus one last-group pass over the 10,840
+    starts.  Returns (K list, seconds per K per core, detail)."""
     import subprocess
     tmp = tempfile.mkdtemp(prefix="ap26cpu")
     ks = W.c3_ks(procs)
+    iters = [0 if i % 2 == 0 else 8 for i in range(len(ks))]
+    per_block = np.cumsum([0] + n43_per_block())
     runs = []
-    for i, K in enumerate(ks):
+    for i, (K, it) in enumerate(zip(ks, iters)):
         d = os.path.join(tmp, str(i))
         os.makedirs(d)
-        runs.append(subprocess.Popen([REF_SAMPLE, str(K), str(W.C3_SHIFT), str(seconds)], cwd=d,
+        runs.append(subprocess.Popen([REF_SAMPLE, str(K), str(W.C3_SHIFT), str(seconds), str(it)], cwd=d,
                                      stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True))
-    rates = []
-    for p in runs:
+    rates = {0: [], 8: []}
+    for p, it in zip(runs, iters):
         out, _ = p.communicate(timeout=seconds * 4 + 120)
         f = out.split()
         blocks, secs = int(f[f.index("blocks") + 1]), float(f[f.index("seconds") + 1])
         if p.returncode != 0 or blocks < 1 or secs <= 0:
             raise RuntimeError("reference CPU sample failed")
-        rates.append(blocks * 4 / 850 / secs)
-    return ks, rates
+        rates[it].append(float(per_block[blocks]) / secs)  # n43 starts per second in this group
+    if not rates[8]:  # one core: time the first group only and price the last group like it
+        rates[8] = rates[0]
+    g1, g3 = float(np.mean(rates[0])), float(np.mean(rates[8]))
+    s_per_k = 2 * 10840 / g1 + 10840 / g3
+    return ks, s_per_k, {"first_group_n43_per_s": g1, "last_group_n43_per_s": g3,
+                         "last_over_first_cost": g1 / g3}
 
 
 def cpu_baseline(n_prp, gpu_prp, j1=2048):
@@ -418,14 +445,15 @@ def cpu_baseline(n_prp, gpu_prp, j1=2048):
     prp = {"tests_per_s_1thread": s.size / dtp, "sample": int(s.size),
            "mismatches_vs_gpu": int((want != gpu_prp[: s.size]).sum()) if gpu_prp is not None else None}
     if os.access(REF_SAMPLE, os.X_OK):
-        ks, rates = reference_cpu_sample(threads)
-        kps = float(sum(rates))
-        return {"value": kps, "unit": "K/s", "cores": threads, "kind": "reference",
-                "cpu_model": model, "cores_available": avail, "s_per_k_per_core": float(np.mean([1 / r for r in rates])),
+        ks, s_per_k, detail = reference_cpu_sample(threads)
+        return {"value": threads / s_per_k, "unit": "K/s", "cores": threads, "kind": "reference",
+                "cpu_model": model, "cores_available": avail, "s_per_k_per_core": s_per_k,
                 "sample": f"the reference CPU build's AVX2 search (Search_avx2, built from its sources), "
                           f"{threads} processes, one per core, each ~20 s of one K (K={ks[0]}..{ks[-1]}, "
-                          f"SHIFT {W.C3_SHIFT}, shifts 0..3) timed at its n43-block progress points",
-                "prp": prp}
+                          f"SHIFT {W.C3_SHIFT}), half in the first shift group (shifts 0-3), half in the "
+                          f"last (shifts 8-9), timed at its n43-block progress points; one K = two "
+                          f"first-group passes + one last-group pass",
+                "detail": detail, "prp": prp}
     t = time.perf_counter()
     O.search_span(W.C3_K0, W.C3_SHIFT, 0, 10, 0, j1, threads=threads)
     dt = time.perf_counter() - t

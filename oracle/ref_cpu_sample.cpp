@@ -8,8 +8,11 @@
 // host driver is compiled with its own renamed), notes the time of every
 // block boundary, and stops after the requested seconds:
 //
-//   ref_cpu_sample K SHIFT SECONDS
-// prints "blocks B seconds T" (B whole blocks of 4/850 K in T seconds).
+//   ref_cpu_sample K SHIFT SECONDS [ITER]
+// prints "blocks B seconds T" (B whole blocks in T seconds).  ITER (default
+// 0) starts the search at shift group ITER/4 like a resumed checkpoint: 8
+// times the last group, whose vectors carry shifts 8 and 9 and two shifts
+// past the K's ten, which the reference masks out.
 // SOL lines the search finds go to ./SOL-AP26.txt, as in the reference.
 #include <chrono>
 #include <cstdio>
@@ -36,7 +39,7 @@ void Progress(double) {
   }
   g_blocks++;
   g_tlast = t;
-  if (t - g_t0 >= g_limit || g_blocks >= 84) { /* stay inside the first group of 4 shifts */
+  if (t - g_t0 >= g_limit || g_blocks >= 84) { /* stay inside the group the search started in */
     printf("blocks %d seconds %.6f\n", g_blocks, g_tlast - g_t0);
     fflush(stdout);
     _exit(0);
@@ -45,12 +48,17 @@ void Progress(double) {
 
 int main(int argc, char **argv) {
   if (argc < 4) {
-    fprintf(stderr, "usage: %s K SHIFT SECONDS\n", argv[0]);
+    fprintf(stderr, "usage: %s K SHIFT SECONDS [ITER]\n", argv[0]);
     return 2;
   }
   const int K = atoi(argv[1]), SHIFT = atoi(argv[2]);
+  const int ITER = argc > 4 ? atoi(argv[4]) : 0;
+  if (ITER != 0 && ITER != 4 && ITER != 8) {
+    fprintf(stderr, "ITER must be 0, 4 or 8 (a shift group boundary)\n");
+    return 2;
+  }
   g_limit = atof(argv[3]);
-  Search_avx2(K, SHIFT, 0, 1, 0);
+  Search_avx2(K, SHIFT, ITER, 1, 0);
   printf("blocks %d seconds %.6f\n", g_blocks, g_tlast - g_t0);
   return 0;
 }
