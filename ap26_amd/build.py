@@ -19,6 +19,7 @@ ORACLE = os.path.join(ROOT, "oracle")
 LIB_GPU = os.path.join(PKG, "libap26gpu.so")
 LIB_HOST = os.path.join(PKG, "libap26host.so")
 BIN_AP26 = os.path.join(PKG, "bin", "AP26")
+BIN_PROBE = os.path.join(PKG, "bin", "ceiling_probe")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -51,6 +52,11 @@ def build_gpu(verbose=False):
     srcs = [s for s in srcs if os.path.exists(s)]
     objs = [os.path.join(PKG, os.path.basename(s) + ".o") for s in srcs]
     cmds = [[HIPCC, *HIP_FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]
+    # the issue/gather ceiling probe (a stand-alone program, not linked into the library)
+    os.makedirs(os.path.dirname(BIN_PROBE), exist_ok=True)
+    probe_src = os.path.join(CSRC, "ceiling_probe.hip")
+    if os.path.exists(probe_src):
+        cmds.append([HIPCC, *HIP_FLAGS, "-o", BIN_PROBE + ".tmp", probe_src])
     if verbose:
         for c in cmds:
             print(" ".join(c))
@@ -68,6 +74,8 @@ def build_gpu(verbose=False):
     for o in objs:
         os.remove(o)
     _atomic_link(tmp, LIB_GPU)
+    if os.path.exists(BIN_PROBE + ".tmp"):
+        _atomic_link(BIN_PROBE + ".tmp", BIN_PROBE)
     return LIB_GPU
 
 

@@ -87,9 +87,10 @@ PEAK_HBM_GBS = 8000.0
 # the sieve issues per searched K of the batched workload, and prp_kernel
 # per modular squaring.  Divided by the launch time they give the VALU
 # instruction rate; VALU_ISSUE_CEILING is the largest rate measured on this
-# GPU, 1.256 wave-instructions per CU-cycle (two-source 32-bit integer
-# instructions at 32 waves per CU; three-source ones and multiplies reach
-# 0.82-0.92), so rate / ceiling is the issue-slot use.  The nominal INT32
+# GPU by ap26_amd/bin/ceiling_probe (profiles/ceiling_probe.md), 1.256
+# wave-instructions per CU-cycle (two-source 32-bit integer instructions at
+# 32 waves per CU; three-source ones and multiplies reach 0.82-0.92), so
+# rate / ceiling is the issue-slot use.  The nominal INT32
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
 # nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
 # sieve's load, so per-clock rates are about 6 % higher).
