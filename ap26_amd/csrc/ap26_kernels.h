@@ -783,6 +783,31 @@ __device__ __forceinline__ void stage_pair_tables(char *lds, const uint64_t *__r
   for (int i = threadIdx.x; i < 180; i += blockDim.x) packed[NPK + i] = tab[2 * NOK + i];
 }
 
+/* a & b & c in one instruction (v_bitop3_b32 with truth table 0x80). */
+__device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t r;
+  asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x80" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+/* AND of a 64-bit mask with N words, two words per three-input instruction
+   and half.  (Written with 64-bit ANDs, the chain compiles to one v_and_b32
+   per word and half.) */
+template <int N>
+__device__ __forceinline__ uint64_t and_words(uint64_t m, const uint64_t (&w)[N]) {
+  uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+  static_for<0, N / 2>([&](auto KC) {
+    constexpr int k = 2 * decltype(KC)::value;
+    lo = and3(lo, (uint32_t)w[k], (uint32_t)w[k + 1]);
+    hi = and3(hi, (uint32_t)(w[k] >> 32), (uint32_t)(w[k + 1] >> 32));
+  });
+  if constexpr (N % 2) {
+    lo &= (uint32_t)w[N - 1];
+    hi &= (uint32_t)(w[N - 1] >> 32);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
 /* The primes [B, E) of a queued (position, shift) entry, in chunks of CH
    primes (residues, loads, ANDs per chunk): with the rotation group's
    constants resident, the deep levels must not hold all their loads at once. */
@@ -814,8 +839,7 @@ __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const c
       }
     });
     if constexpr (E <= NL) interleave_residues_and_reads<E - B>();
-    static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
-    return m;
+    return and_words(m, w);
   }
 }
 
