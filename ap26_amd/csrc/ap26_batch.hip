@@ -32,6 +32,7 @@ using PersistFn = void (*)(PersistArgs);
 struct PersistVariant {
   int NL, waves, S;
   PersistFn fn;
+  int gq_entries = 0; /* global queue entries per wave (PersistArgs::gq) */
 };
 
 template <int NL, int WAVES, class LV>
@@ -50,17 +51,27 @@ PersistVariant make_pvariant2r() {
   return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
 }
 
+/* the rotated form with queues GQ.. in global memory, so that more primes fit in LDS */
+template <int NL, int WAVES, class LV, class TU, int GQ>
+PersistVariant make_pvariant2rg() {
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU, GQ>,
+                        qsieve2_global_entries<LV, GQ>()};
+}
+
 /* 1..3, 9: one shift per item; 10..40, 50..56: shift pairs.  Level 0
    (every position, both shifts of a pair per 16-byte read) runs on the first
    16..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
    so level 0 pays off further than on one shift (61..131).  35..38: the first
    1..3 level-0 primes from register bit windows; 39..40: the last 2 or 4
-   level-0 primes read through L1 instead of LDS.  50..74 (0 = 70, the
-   default): level 0 opens with the bank-rotated group of 61..137
-   (level0_rot), whose 64 VGPRs of per-lane constants leave room for 12 waves
-   per CU (16 waves spill in the hot loop, 8 lose the latency hiding); 57..74
-   vary the level boundaries, the wave priorities and how many rotated
-   gathers are in flight at once (Tune). */
+   level-0 primes read through L1 instead of LDS.  50..91: level 0 opens
+   with the bank-rotated group of 61..137 (level0_rot), whose 64 VGPRs of
+   per-lane constants leave room for 12 waves per CU (16 waves spill in the
+   hot loop, 8 lose the latency hiding); 57..74 vary the level boundaries,
+   the wave priorities and how many rotated gathers are in flight at once
+   (Tune).  75..91 move the deeper queues to global memory and spend the
+   LDS on more table primes: a level that reads its tables through L1/L2
+   costs more than the queue traffic (0 = 85, the default: the last queue
+   in global memory, primes 61..251 in LDS). */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -124,8 +135,27 @@ PersistVariant pick_pvariant(int id) {
     case 72: return make_pvariant2r<30, 12, Levels<0, 19, 30, 49, 83>, Tune<2, 3, 1, 1>>();
     case 73: return make_pvariant2r<29, 12, Levels<0, 19, 29, 51, 83>, Tune<2, 3, 1, 1>>();
     case 74: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<3, 3, 1, 1>>();
+    /* 75..80: queues 1.. in global memory; 42 or 38 primes in LDS */
+    case 75: return make_pvariant2rg<42, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 76: return make_pvariant2rg<42, 12, Levels<0, 20, 30, 42, 60, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 77: return make_pvariant2rg<42, 12, Levels<0, 20, 32, 42, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 78: return make_pvariant2rg<42, 12, Levels<0, 20, 42, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 79: return make_pvariant2rg<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 80: return make_pvariant2rg<38, 12, Levels<0, 20, 30, 38, 55, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 81..85: only the last queue in global memory; 37 primes in LDS */
+    case 81: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 82: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 37, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 83: return make_pvariant2rg<37, 12, Levels<0, 20, 37, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 84: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 44, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 85: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 86: return make_pvariant2rg<37, 12, Levels<0, 20, 34, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 87: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 55, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 88: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 89: return make_pvariant2rg<37, 12, Levels<0, 19, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 90: return make_pvariant2rg<37, 12, Levels<0, 20, 32, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 91: return make_pvariant2rg<37, 12, Levels<0, 21, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
-    default: return make_pvariant2r<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>(); /* = 70 */
+    default: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>(); /* = 85 */
   }
 }
 
@@ -164,6 +194,7 @@ struct BatchState {
   uint64_t *d_tab = nullptr;
   int64_t *d_surv = nullptr;
   int32_t *d_surv_job = nullptr;
+  QEntry *d_gq = nullptr;     /* global queue regions: cu_count x waves x pv.gq_entries */
   std::vector<hipEvent_t> ev; /* 2 per timed round */
 };
 
@@ -192,6 +223,7 @@ static void batch_state_free(BatchState *b) {
   (void)hipFree(b->d_tab);
   (void)hipFree(b->d_surv);
   (void)hipFree(b->d_surv_job);
+  (void)hipFree(b->d_gq);
   delete b;
 }
 
@@ -239,6 +271,10 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, int nio, BatchState **o
               hipMalloc(&b->d_tab, sizeof(uint64_t) * NTAB * AP26_NSHIFT * b->per_round) != hipSuccess ||
               hipMalloc(&b->d_surv, sizeof(int64_t) * b->surv_cap) != hipSuccess ||
               hipMalloc(&b->d_surv_job, sizeof(int32_t) * b->surv_cap) != hipSuccess))
+    rc = AP26_E_NOMEM;
+  /* one region per wave of the one-workgroup-per-CU launch (enqueue_round) */
+  if (!rc && b->pv.gq_entries > 0 &&
+      hipMalloc(&b->d_gq, sizeof(QEntry) * (size_t)ctx->cu_count * b->pv.waves * b->pv.gq_entries) != hipSuccess)
     rc = AP26_E_NOMEM;
   if (!rc && hipMemcpy(b->d_triples, trip.data(), trip.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
     rc = AP26_E_DEVICE;
@@ -333,6 +369,7 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part, const RoundIO &
     p.err = (unsigned int *)(o.d_cnt + 6);
     p.chunk = b->chunk;
     p.nchunk = b->nchunk;
+    p.gq = b->d_gq; /* sized for dim3(ctx->cu_count) workgroups of pv.waves waves */
     hipLaunchKernelGGL(b->pv.fn, dim3(ctx->cu_count), dim3(64 * b->pv.waves), 0, ctx->stream, p);
     return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
   }
@@ -1039,7 +1076,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {70, 67, 58, 54, 28, 53, 55, 32, 10};
+static const int TUNE_VARIANTS[] = {85, 86, 75, 70, 67, 58, 54, 28, 10};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -34,6 +34,7 @@ using SieveFn = void (*)(SieveArgs);
 struct SieveVariant {
   int S, NL, block;
   SieveFn fn;
+  int gq_entries = 0; /* global queue entries per wave (SieveArgs::gq) */
 };
 
 template <int S, int NL>
@@ -57,6 +58,13 @@ SieveVariant make_q2variantr() {
   return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
 }
 
+/* the same with queues GQ.. in global memory, so that more primes fit in LDS */
+template <int NL, int WAVES, class LV, class TU, int GQ>
+SieveVariant make_q2variantrg() {
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU, GQ>,
+                      qsieve2_global_entries<LV, GQ>()};
+}
+
 
 struct BatchState;
 struct ap26_ctx {
@@ -75,7 +83,9 @@ struct ap26_ctx {
   int64_t surv_cap = 0;
   unsigned long long *d_cnt = nullptr; /* NCOUNT counters (sol count stored as u32 at C_NSOL) */
   DevSol *d_sols = nullptr;      /* SOL_CAP */
+  void *d_gq = nullptr;          /* the sieve variant's global queues: grid x waves regions */
   SieveVariant sieve{};
+  SieveVariant sieve_lds{};      /* all queues in LDS: the allocation-free entry point's sieve */
   BatchState *batch = nullptr;
   int tune_variant = -1; /* batched sieve variant / work-item chunk chosen by */
   int tune_chunk = -1;   /* ap26_autotune (or its cache); -1 = default       */

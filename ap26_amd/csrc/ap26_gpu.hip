@@ -64,7 +64,9 @@ SieveVariant pick_variant(int id) {
     case 43: return make_q2variantr<36, 8, Levels<0, 18, 36, 55, 83>>();
     case 44: return make_q2variantr<21, 16, Levels<0, 16, 21, 51, 83>>();
     case 45: return make_q2variantr<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>();
-    default: return make_q2variantr<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>(); /* = 45 */
+    /* 46: the last queue in global memory, 37 primes in LDS */
+    case 46: return make_q2variantrg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    default: return make_q2variantrg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>(); /* = 46 */
   }
 }
 
@@ -160,9 +162,17 @@ int ap26_open(int device, const ap26_opts *opts, ap26_ctx **out) {
     ctx->surv_cap = SURV_CAP;
     if ((rc = scratch_address(device, &ctx->scratch)) != AP26_OK) break;
     ctx->sieve = pick_variant(ctx->opts.sieve_variant);
+    ctx->sieve_lds = ctx->sieve.gq_entries > 0 ? pick_variant(45) : ctx->sieve;
     tune_load_cache(ctx);
     ctx->opts.block = ctx->sieve.block;
     if (ctx->opts.blocks_per_cu <= 0) ctx->opts.blocks_per_cu = ctx->sieve.block > 256 ? 1 : 2;
+    /* one global queue region per wave of enqueue_sieve's grid */
+    if (ctx->sieve.gq_entries > 0 &&
+        hipMalloc(&ctx->d_gq, sizeof(QEntry) * (size_t)ctx->cu_count * ctx->opts.blocks_per_cu *
+                                  (ctx->sieve.block / 64) * ctx->sieve.gq_entries) != hipSuccess) {
+      rc = AP26_E_NOMEM;
+      break;
+    }
   } while (0);
   if (rc) {
     ap26_close(ctx);
@@ -181,6 +191,7 @@ int ap26_close(ap26_ctx *ctx) {
   (void)hipFree(ctx->d_surv);
   (void)hipFree(ctx->d_cnt);
   (void)hipFree(ctx->d_sols);
+  (void)hipFree(ctx->d_gq);
   batch_free(ctx);
   for (auto &ev : ctx->ev)
     if (ev) (void)hipEventDestroy(ev);
@@ -219,17 +230,21 @@ struct SpanBufs {
   unsigned long long *cnt;   /* NCOUNT counters */
   DevSol *sols;
   int32_t sol_cap;
+  const SieveVariant *sv;    /* the sieve to launch */
+  void *gq;                  /* its global queues (sv->gq_entries > 0) */
 };
 
 static SpanBufs ctx_bufs(ap26_ctx *ctx) {
-  return SpanBufs{ctx->d_tab, ctx->d_n43, ctx->d_surv, ctx->surv_cap, ctx->d_cnt, ctx->d_sols, SOL_CAP};
+  return SpanBufs{ctx->d_tab, ctx->d_n43, ctx->d_surv, ctx->surv_cap, ctx->d_cnt, ctx->d_sols, SOL_CAP,
+                  &ctx->sieve, ctx->d_gq};
 }
 
 /* Enqueue sieve of shifts [it0,it1) x n43 [j0,j1) into bufs.surv (counter
    C_NSURV).  Counters must have been cleared. */
 static int enqueue_sieve(ap26_ctx *ctx, const SpanBufs &bf, const KParams &kp, int it0, int it1, int j0, int j1,
                          ap26_stats *st) {
-  const int S = ctx->sieve.S;
+  const SieveVariant &sv = *bf.sv;
+  const int S = sv.S;
   const int nslot = it1 - it0;
   const int npass = (nslot + S - 1) / S;
   /* tables for every shift slot of every pass */
@@ -254,7 +269,8 @@ static int enqueue_sieve(ap26_ctx *ctx, const SpanBufs &bf, const KParams &kp, i
     a.job = 0;
     a.err = (unsigned int *)(bf.cnt + C_ERR);
     for (int v = 0; v < NVW; v++) a.vc[v] = vw_offset(a.shift0, VW_P[v]);
-    hipLaunchKernelGGL(ctx->sieve.fn, dim3(grid), dim3(ctx->sieve.block), 0, ctx->stream, a);
+    a.gq = bf.gq; /* sized for `grid` workgroups of sv.block threads (ap26_open) */
+    hipLaunchKernelGGL(sv.fn, dim3(grid), dim3(sv.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
   }
@@ -433,7 +449,7 @@ int ap26_search_span_noalloc(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it
   unsigned char *base = (unsigned char *)ctx->scratch;
   const SpanBufs bf{(uint64_t *)(base + NA_TAB), (int64_t *)(base + NA_N43), (int64_t *)(base + NA_SURV),
                     NOALLOC_SURV, (unsigned long long *)(base + NA_CNT), (DevSol *)(base + NA_SOLS),
-                    NOALLOC_SOLS};
+                    NOALLOC_SOLS, &ctx->sieve_lds, nullptr}; /* every sieve queue in LDS */
   KParams kp;
   HostK hk;
   host_kconsts(K, &hk);

@@ -117,6 +117,7 @@ struct SieveArgs {
   int32_t job;
   unsigned int *__restrict__ err;   /* set if a queue step ran on part of a wave */
   int32_t vc[NVW];                  /* (shift0 mod p) + p*((p+1)/2), p = 61, 67, 71 */
+  void *__restrict__ gq;            /* QEntry regions, one per wave of the grid (variants with GQ) */
 };
 
 __host__ __device__ constexpr int32_t vw_offset(int32_t shift0, int p) { return shift0 % p + p * ((p + 1) / 2); }
@@ -538,6 +539,7 @@ struct PersistArgs {
   unsigned int *__restrict__ err;    /* queue steps on part of a wave */
   int32_t chunk;                     /* n43 starts per item */
   int32_t nchunk;
+  QEntry *__restrict__ gq;           /* per-wave global queue regions (variants with GQ) */
 };
 
 /* Work items (job, shift, n43 chunk) of one round, taken from a global atomic
@@ -1069,17 +1071,32 @@ struct Tune {
 /* Queue capacity per wave: queue 0 holds Q0 entries, the deeper ones 128.
    With Q0 = 192 both pushes of a position (one per shift of the pair) go in
    before one drain: a single call site of the drain chain per position
-   (its code and the registers it needs appear once). */
-template <int Q0, class LV>
+   (its code and the registers it needs appear once).  Queues GQ and deeper
+   live in a per-wave region of global memory instead of LDS (GQ >= the queue
+   count: all in LDS); the LDS holds the others. */
+constexpr int NO_GLOBAL_QUEUES = 99;
+
+template <int Q0, class LV, int GQ = NO_GLOBAL_QUEUES>
 constexpr int qsieve2_wave_entries() {
-  return Q0 + 128 * (LV::N - 2);
+  constexpr int NQ = LV::N - 1;
+  constexpr int NL_Q = NQ < GQ ? NQ : GQ; /* queues in LDS */
+  return Q0 + 128 * (NL_Q - 1);
 }
 
-template <int NL, class LV, int R0 = 0, int Q0 = 128, class TU = Tune<>>
+/* Global-memory queue entries per wave (queues GQ .. NQ-1, 128 each). */
+template <class LV, int GQ>
+constexpr int qsieve2_global_entries() {
+  constexpr int NQ = LV::N - 1;
+  return GQ < NQ ? 128 * (NQ - GQ) : 0;
+}
+
+template <int NL, class LV, int R0 = 0, int Q0 = 128, class TU = Tune<>, int GQ = NO_GLOBAL_QUEUES>
 struct QSieve2 {
   static_assert(Q0 == 128 || Q0 == 192, "queue 0 holds one or two pushes past a full wave");
+  static_assert(GQ >= 1, "queue 0 stays in LDS");
   static constexpr int NQ = LV::N - 1;
-  QEntry *qb; /* this wave's queues: queue 0 (Q0 entries), then 128 per level */
+  QEntry *qb;  /* this wave's LDS queues: queue 0 (Q0 entries), then 128 per level below GQ */
+  QEntry *gqb; /* this wave's global queues GQ.. (128 entries each) */
   int qn[NQ];
   bool partial; /* a queue step ran with lanes switched off (must never happen) */
   const char *ldsb, *gtb;
@@ -1096,12 +1113,23 @@ struct QSieve2 {
     partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
   }
 
-  __device__ __forceinline__ QEntry *q(int qi) const { return qb + (qi == 0 ? 0 : Q0 + 128 * (qi - 1)); }
+  /* A wave reads back its own global queue entries, written by other lanes
+     of the same wave: its vector memory operations are processed in order,
+     and the CU's L1 is write-through, so no fence is needed between a push
+     and the drain that pops it. */
+  template <int QI>
+  __device__ __forceinline__ QEntry *q() const {
+    if constexpr (QI >= GQ)
+      return gqb + 128 * (QI - GQ);
+    else
+      return qb + (QI == 0 ? 0 : Q0 + 128 * (QI - 1));
+  }
 
-  __device__ __forceinline__ void push(int qi, double xd, uint64_t m, bool pred) {
+  template <int QI>
+  __device__ __forceinline__ void push(double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
-    if (pred) q(qi)[qn[qi] + lanes_below(bal)] = QEntry{xd, m};
-    qn[qi] += __popcll(bal);
+    if (pred) q<QI>()[qn[QI] + lanes_below(bal)] = QEntry{xd, m};
+    qn[QI] += __popcll(bal);
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -1113,7 +1141,7 @@ struct QSieve2 {
       if (pred && m) m = apply_filters2<NL, R0>(m, e.xd, ldsb);
       emit_survivors2(pred ? m : 0, e.xd, *a, lane, *cur);
     } else {
-      push(L + 1, e.xd, m, pred && m != 0);
+      push<L + 1>(e.xd, m, pred && m != 0);
       drain<L + 1>();
     }
   }
@@ -1123,7 +1151,7 @@ struct QSieve2 {
     while (qn[L] >= 64) {
       check_full_wave();
       qn[L] -= 64;
-      const QEntry e = q(L)[qn[L] + lane];
+      const QEntry e = q<L>()[qn[L] + lane];
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_s_setprio(L == 0 ? TU::prio1 : TU::prio_deep);
       run_level<L>(e, true);
@@ -1140,7 +1168,7 @@ struct QSieve2 {
         check_full_wave();
         const bool pred = lane < n;
         QEntry e{TWO52, 0};
-        if (pred) e = q(L)[lane];
+        if (pred) e = q<L>()[lane];
         __builtin_amdgcn_wave_barrier();
         qn[L] = 0;
         run_level<L>(e, pred);
@@ -1152,8 +1180,9 @@ struct QSieve2 {
   }
 };
 
-template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>>
-__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU> &qs, const SieveArgs &a, int64_t tid,
+template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>,
+          int GQ = NO_GLOBAL_QUEUES>
+__device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride, const RotConsts *rc = nullptr) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
@@ -1175,9 +1204,9 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU> &qs, 
       __builtin_amdgcn_s_setprio(0);
       m0 = has ? m0 : 0;
       m1 = has ? m1 : 0;
-      qs.push(0, xd, m0, m0 != 0);
+      qs.template push<0>(xd, m0, m0 != 0);
       if constexpr (Q0 == 128) qs.template drain<0>();
-      qs.push(0, xd + TWO48, m1, m1 != 0);
+      qs.template push<0>(xd + TWO48, m1, m1 != 0);
       qs.template drain<0>();
       xd += S59d;
       xd = xd >= WRAP ? xd - MODd : xd;
@@ -1186,30 +1215,31 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU> &qs, 
   qs.template flush<0>();
 }
 
-template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128>
+template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128, int GQ = NO_GLOBAL_QUEUES>
 struct QSieve2Lds {
   /* rotation rows (R0 = ROT_N), packed pair entries of primes R0..NL-1, filters */
   alignas(16) char tab[rot_lds_bytes<R0>() + 16 * (PT.off[NL] - PT.off[R0] + 90)];
-  QEntry queues[WAVES][qsieve2_wave_entries<Q0, LV>()];
+  QEntry queues[WAVES][qsieve2_wave_entries<Q0, LV, GQ>()];
 };
 
-template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128>
+template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128, int GQ = NO_GLOBAL_QUEUES>
 constexpr int qsieve2_lds_bytes() {
-  return (int)sizeof(QSieve2Lds<NL, WAVES, LV, R0, Q0>);
+  return (int)sizeof(QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ>);
 }
 
 /* Items (job, shift pair, n43 chunk); tables [entry][shift in pair].  V: the
    first V level-0 primes from registers (vwin_pair) instead of LDS.  R0 =
    ROT_N: level 0 starts with the bank-rotated group (level0_rot). */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>,
+          int GQ = NO_GLOBAL_QUEUES>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_persistent2_kernel(PersistArgs p) {
   constexpr int Q0 = R0 ? 192 : 128;
-  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0>() + 64 + 16 * WAVES <= 160 * 1024, "LDS budget");
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0, GQ>() + 64 + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
   static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
   constexpr int NPASS = AP26_NSHIFT / 2;
-  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0> L;
+  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ> L;
   __shared__ unsigned int s_item;
   const RoundState rs = *p.rs;
   const int njobs = min(rs.per_round, rs.njobs - rs.base);
@@ -1259,30 +1289,35 @@ sieve_persistent2_kernel(PersistArgs p) {
     a.surv_job = p.surv_job;
     a.err = p.err;
     a.job = jl;
-    QSieve2<NL, LV, R0, Q0, TU> qs;
+    QSieve2<NL, LV, R0, Q0, TU, GQ> qs;
     qs.lane = threadIdx.x & 63;
     qs.qb = L.queues[threadIdx.x >> 6];
-    for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU>::NQ; i++) qs.qn[i] = 0;
+    /* p.gq holds gridDim.x * WAVES regions of qsieve2_global_entries entries;
+       the wave index is uniform, so the global base stays in scalar registers */
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    qs.gqb = p.gq + ((size_t)blockIdx.x * WAVES + wave) * qsieve2_global_entries<LV, GQ>();
+    for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU, GQ>::NQ; i++) qs.qn[i] = 0;
     qs.partial = false;
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU>(qs, a, threadIdx.x, blockDim.x, &rc);
+    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ>(qs, a, threadIdx.x, blockDim.x, &rc);
   }
 }
 
 /* One launch per (K, shift pair): the plain path with the shift-pair sieve
    (tables [entry][shift in pair], level 0 with one 16-byte read per prime for
    both shifts), survivors in per-wave slot chunks as in the batched engine. */
-template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>>
+template <int NL, int WAVES, class LV, int V = 0, int G = 0, int R0 = 0, class TU = Tune<>,
+          int GQ = NO_GLOBAL_QUEUES>
 static __global__ void __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(WAVES / 4, (WAVES + 3) / 4)))
 sieve_queue2_kernel(SieveArgs a) {
   constexpr int Q0 = R0 ? 192 : 128;
-  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0>() + 16 * WAVES <= 160 * 1024, "LDS budget");
+  static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0, GQ>() + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
   static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
-  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0> L;
+  __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ> L;
   __shared__ SurvCursor s_cur[WAVES];
   stage_pair_tables<R0, NL>(L.tab, a.tab);
   SurvCursor &sc = s_cur[threadIdx.x >> 6];
@@ -1290,16 +1325,19 @@ sieve_queue2_kernel(SieveArgs a) {
   RotConsts rc;
   if constexpr (R0) rc.init(threadIdx.x & 63);
   __syncthreads();
-  QSieve2<NL, LV, R0, Q0, TU> qs;
+  QSieve2<NL, LV, R0, Q0, TU, GQ> qs;
   qs.lane = threadIdx.x & 63;
   qs.qb = L.queues[threadIdx.x >> 6];
-  for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU>::NQ; i++) qs.qn[i] = 0;
+  /* a.gq holds gridDim.x * WAVES regions of qsieve2_global_entries entries */
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  qs.gqb = (QEntry *)a.gq + ((size_t)blockIdx.x * WAVES + wave) * qsieve2_global_entries<LV, GQ>();
+  for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU, GQ>::NQ; i++) qs.qn[i] = 0;
   qs.partial = false;
   qs.ldsb = (const char *)L.tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV, V, G, R0, Q0, TU>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+  qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
                                            (int64_t)gridDim.x * blockDim.x, &rc);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }
