@@ -1076,7 +1076,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {85, 86, 75, 70, 67, 58, 54, 28, 10};
+static const int TUNE_VARIANTS[] = {85, 86, 90, 88, 70, 75, 67, 58, 54, 28, 10};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -94,15 +94,16 @@ PEAK_HBM_GBS = 8000.0
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
 # nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
 # sieve's load, so per-clock rates are about 6 % higher).
-SIEVE_VALU_LANE_INSTR_PER_K = 4.4769e12
+SIEVE_VALU_LANE_INSTR_PER_K = 4.2603e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 197,328 + 11,496,800) KiB / 16 K.
-TRAFFIC_BYTES_PER_K = 7.611e8
+# (profiles/sieve_rot_traffic.md): (2 x 254,429 + 54,309,600) KiB / 16 K,
+# most of it the default kernel's global-memory queue written back from L2.
+TRAFFIC_BYTES_PER_K = 3.508e9
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
 # n43 starts (10,840 x 8 B) read once.
@@ -480,7 +481,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
-    ap.add_argument("--full-tune", action="store_true", help="autotune over all 27 candidates (9 variants x 3 work-item sizes)")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 33 candidates (11 variants x 3 work-item sizes)")
     ap.add_argument("--serve-loads", default="0.25,0.5,1.0",
                     help="offered loads of the serving sweep, as fractions of the saturated serving K/s")
     ap.add_argument("--serve-units", type=int, default=10)
