@@ -51,6 +51,13 @@ PersistVariant make_pvariant2r() {
   return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
 }
 
+/* the rotated form with 512-byte rows (no per-lane column registers) */
+template <int NL, int WAVES, class LV, class TU, int GQ>
+PersistVariant make_pvariant2rw() {
+  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_WIDE, TU, GQ>,
+                        qsieve2_global_entries<LV, GQ>()};
+}
+
 /* the rotated form with queues GQ.. in global memory, so that more primes fit in LDS */
 template <int NL, int WAVES, class LV, class TU, int GQ>
 PersistVariant make_pvariant2rg() {
@@ -71,7 +78,9 @@ PersistVariant make_pvariant2rg() {
    (Tune).  75..91 move the deeper queues to global memory and spend the
    LDS on more table primes: a level that reads its tables through L1/L2
    costs more than the queue traffic (0 = 85, the default: the last queue
-   in global memory, primes 61..251 in LDS). */
+   in global memory, primes 61..251 in LDS).  92..102 use the 512-byte
+   rotation rows (ROT_WIDE: no per-lane column registers or adds, twice the
+   LDS); 16 waves still spill the rotation constants in the hot loop. */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -154,6 +163,18 @@ PersistVariant pick_pvariant(int id) {
     case 89: return make_pvariant2rg<37, 12, Levels<0, 19, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
     case 90: return make_pvariant2rg<37, 12, Levels<0, 20, 32, 51, 83>, Tune<2, 3, 1, 1>, 2>();
     case 91: return make_pvariant2rg<37, 12, Levels<0, 21, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    /* 92..97: 512-byte rotation rows; 16 waves where the registers allow */
+    case 92: return make_pvariant2rw<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 93: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 2>, 1>();
+    case 94: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 95: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 96: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, Tune<2, 3, 1, 4>, 1>();
+    case 97: return make_pvariant2rw<27, 12, Levels<0, 20, 27, 51, 83>, Tune<2, 3, 1, 1>, 2>();
+    case 98: return make_pvariant2rw<34, 12, Levels<0, 20, 34, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 99: return make_pvariant2rw<34, 12, Levels<0, 20, 32, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 100: return make_pvariant2rw<34, 12, Levels<0, 20, 34, 55, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 101: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 102: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     default: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>(); /* = 85 */
   }
@@ -1076,7 +1097,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {85, 86, 90, 88, 70, 75, 67, 58, 54, 28, 10};
+static const int TUNE_VARIANTS[] = {85, 86, 90, 88, 70, 95, 75, 67, 58, 54, 28, 10};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -735,17 +735,27 @@ constexpr int ROT_ROWS = 2 * ROT_HALF + 1;          /* 137 */
 constexpr int ROT_BYTES = 256 * ROT_ROWS;           /* 35,072 */
 static_assert(ROT_N == 16 && PT.p[ROT_N - 1] == 137, "rotation group is 61..137");
 
-/* LDS bytes in front of the packed shift-pair tables, and the first prime
-   index the packed tables hold: R0 = 0 (no rotation group) or ROT_N. */
+/* The rotation layouts, chosen by R0: 0 (none), ROT_N (rows of 256 bytes,
+   one column per prime) or ROT_WIDE (rows of 512 bytes holding every column
+   twice: at step k lane j reads column (j mod 16) + k, whose bank group is
+   (j + k) mod 16 as before, but whose offset 16 (j mod 16) + 16 k is a
+   per-lane constant plus an immediate, so the position's multiply-add
+   produces the address and no per-lane column registers are needed). */
+constexpr int ROT_WIDE = 2 * ROT_N;
+constexpr int rot_primes(int R0) { return R0 ? ROT_N : 0; }            /* first packed prime */
+constexpr int rot_row_bytes(int R0) { return R0 == ROT_WIDE ? 512 : 256; }
+
+/* LDS bytes in front of the packed shift-pair tables. */
 template <int R0>
 constexpr int rot_lds_bytes() {
-  return R0 ? ROT_BYTES : 0;
+  return R0 ? rot_row_bytes(R0) * ROT_ROWS : 0;
 }
 
-struct RotConsts {
+template <int W>
+struct RotConstsT {
   double inv[ROT_N]; /* RN(1/p), the constant centred_byteoff uses */
-  int m[ROT_N];      /* -256 p */
-  int col[ROT_N];    /* 16 q: byte offset of prime q's column in a row */
+  int m[ROT_N];      /* -W p */
+  int col[ROT_N];    /* 16 q: byte offset of prime q's column in a 256-byte row */
   __device__ __forceinline__ void init(int lane) {
     static_for<0, ROT_N>([&](auto KC) {
       constexpr int k = decltype(KC)::value;
@@ -756,11 +766,12 @@ struct RotConsts {
         p = q == Q ? PT.p[Q] : p;
       });
       inv[k] = 1.0 / (double)p; /* correctly rounded, = the compile-time 1.0/P */
-      m[k] = -256 * p;
+      m[k] = -W * p;
       col[k] = 16 * q;
     });
   }
 };
+using RotConsts = RotConstsT<256>;
 
 /* Stage one shift pair's tables into LDS: the rotation rows (R0 = ROT_N),
    the packed entries of primes R0..NL-1 and the 7..23 filters.  tab is the
@@ -769,19 +780,21 @@ template <int R0, int NL>
 __device__ __forceinline__ void stage_pair_tables(char *lds, const uint64_t *__restrict__ tab) {
   constexpr int RB = rot_lds_bytes<R0>();
   if constexpr (R0) {
+    constexpr int NC = rot_row_bytes(R0) / 16; /* columns per row: 16 or 32 */
     const uint4 *t4 = (const uint4 *)tab;
     uint4 *rows = (uint4 *)lds;
-    for (int i = threadIdx.x; i < ROT_ROWS * ROT_N; i += blockDim.x) {
+    for (int i = threadIdx.x; i < ROT_ROWS * NC; i += blockDim.x) {
       const int q = i & (ROT_N - 1);
       const int p = PT.p[q];
-      int c = (i >> 4) - ROT_HALF + (p - 1) / 2; /* centred entry of residue row - ROT_HALF */
+      int c = i / NC - ROT_HALF + (p - 1) / 2; /* centred entry of residue row - ROT_HALF */
       c = (c % p + p) % p;                      /* rows past (p-1)/2 are never read; keep them valid */
       rows[i] = t4[PT.off[q] + c];
     }
   }
   uint64_t *packed = (uint64_t *)(lds + RB);
-  constexpr int NPK = 2 * (PT.off[NL] - PT.off[R0]);
-  for (int i = threadIdx.x; i < NPK; i += blockDim.x) packed[i] = tab[2 * PT.off[R0] + i];
+  constexpr int P0 = rot_primes(R0);
+  constexpr int NPK = 2 * (PT.off[NL] - PT.off[P0]);
+  for (int i = threadIdx.x; i < NPK; i += blockDim.x) packed[i] = tab[2 * PT.off[P0] + i];
   for (int i = threadIdx.x; i < 180; i += blockDim.x) packed[NPK + i] = tab[2 * NOK + i];
 }
 
@@ -815,7 +828,7 @@ __device__ __forceinline__ uint64_t and_words(uint64_t m, const uint64_t (&w)[N]
    constants resident, the deep levels must not hold all their loads at once. */
 template <int NL, int B, int E, int R0 = 0, int CH = 64>
 __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const char *ldsb, const char *gtb) {
-  static_assert(B >= R0, "rotated primes are level 0 only");
+  static_assert(B >= rot_primes(R0), "rotated primes are level 0 only");
   double x;
   uint32_t xs;
   int e;
@@ -833,7 +846,7 @@ __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const c
     static_for<B, E>([&](auto IC) {
       constexpr int I = decltype(IC)::value;
       if constexpr (I < NL) {
-        constexpr int C0 = rot_lds_bytes<R0>() + 16 * (PT.off[I] - PT.off[R0] + (PT.p[I] - 1) / 2);
+        constexpr int C0 = rot_lds_bytes<R0>() + 16 * (PT.off[I] - PT.off[rot_primes(R0)] + (PT.p[I] - 1) / 2);
         w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
       } else {
         constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
@@ -851,7 +864,7 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
   uint32_t xs;
   int e;
   decode2(xd, x, xs, e);
-  constexpr int LMAIN = PT.off[NL] - PT.off[R0];
+  constexpr int LMAIN = PT.off[NL] - PT.off[rot_primes(R0)];
   static_for<NSP, NPT>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     const int bo = centred_byteoff<PT.p[I], 16>(x, xs);
@@ -864,14 +877,17 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
 /* Level 0 of a fresh position with the rotation group: both masks of the
    pair over primes 0..E-1, the first ROT_N through the rotated rows, the rest
    (ROT_N..E-1) from the packed LDS tables. */
-template <int E, int SPLIT = 2>
-__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConsts &rc, uint64_t &m0,
+template <int E, int SPLIT = 2, int W = 256>
+__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConstsT<W> &rc, uint64_t &m0,
                                            uint64_t &m1) {
   static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
+  static_assert(W == 256 || W == 512, "row width");
   constexpr int PART = ROT_N / SPLIT;
   static_assert(E >= ROT_N, "level 0 holds the rotation group");
   const double x = xd - TWO52;
-  const uint32_t xs256 = lo32(xd) * 256u;
+  /* W * x mod 2^32 (exact: W * 2^24 = 0 mod 2^32 for the i24 multiply-add);
+     wide rows: plus the row offset of residue 0 and this lane's column */
+  const uint32_t xsw = lo32(xd) * (uint32_t)W + (W == 512 ? 512u * ROT_HALF + 16u * (threadIdx.x & 15) : 0u);
   const uint32_t xs16 = lo32(xd) * 16u;
   uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
   /* SPLIT parts of the group, so that at most 16/SPLIT gathers are in flight */
@@ -881,9 +897,12 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
     static_for<0, PART>([&](auto KC) {
       constexpr int k = H * PART + decltype(KC)::value;
       const double qd = __fma_rn(x, rc.inv[k], SHIFTER);
-      int r; /* 256 * (centred residue of prime (lane + k) mod 16) */
-      asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"((int)lo32(qd)), "v"(rc.m[k]), "v"(xs256));
-      w[decltype(KC)::value] = *(const uint4 *)(ldsb + 256 * ROT_HALF + r + rc.col[k]);
+      int r; /* W * (centred residue of prime (lane + k) mod 16) [+ the wide row's offsets] */
+      asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"((int)lo32(qd)), "v"(rc.m[k]), "v"(xsw));
+      if constexpr (W == 512)
+        w[decltype(KC)::value] = *(const uint4 *)(ldsb + r + 16 * k); /* column (lane mod 16) + k */
+      else
+        w[decltype(KC)::value] = *(const uint4 *)(ldsb + 256 * ROT_HALF + r + rc.col[k]);
     });
     interleave_residues_and_reads<PART>();
     static_for<0, PART>([&](auto KC) {
@@ -899,7 +918,7 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
     static_for<ROT_N, E>([&](auto IC) {
       constexpr int I = decltype(IC)::value;
       const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
-      constexpr int C0 = ROT_BYTES + 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
+      constexpr int C0 = W * ROT_ROWS + 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
       w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
     });
     static_for<0, E - ROT_N>([&](auto IC) {
@@ -1183,7 +1202,8 @@ struct QSieve2 {
 template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>,
           int GQ = NO_GLOBAL_QUEUES>
 __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &qs, const SieveArgs &a, int64_t tid,
-                                               int64_t stride, const RotConsts *rc = nullptr) {
+                                               int64_t stride,
+                                               const RotConstsT<rot_row_bytes(R0)> *rc = nullptr) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
@@ -1198,7 +1218,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(TU::prio0);
       if constexpr (R0)
-        level0_rot<LV::B[1], TU::split>(xd, qs.ldsb, *rc, m0, m1);
+        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0)>(xd, qs.ldsb, *rc, m0, m1);
       else
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
       __builtin_amdgcn_s_setprio(0);
@@ -1218,7 +1238,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
 template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128, int GQ = NO_GLOBAL_QUEUES>
 struct QSieve2Lds {
   /* rotation rows (R0 = ROT_N), packed pair entries of primes R0..NL-1, filters */
-  alignas(16) char tab[rot_lds_bytes<R0>() + 16 * (PT.off[NL] - PT.off[R0] + 90)];
+  alignas(16) char tab[rot_lds_bytes<R0>() + 16 * (PT.off[NL] - PT.off[rot_primes(R0)] + 90)];
   QEntry queues[WAVES][qsieve2_wave_entries<Q0, LV, GQ>()];
 };
 
@@ -1237,7 +1257,8 @@ sieve_persistent2_kernel(PersistArgs p) {
   constexpr int Q0 = R0 ? 192 : 128;
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0, GQ>() + 64 + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
-  static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
+  static_assert(R0 == 0 || ((R0 == ROT_N || R0 == ROT_WIDE) && V == 0 && G == 0),
+                "rotation group replaces V/G level-0 forms");
   constexpr int NPASS = AP26_NSHIFT / 2;
   __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ> L;
   __shared__ unsigned int s_item;
@@ -1248,7 +1269,7 @@ sieve_persistent2_kernel(PersistArgs p) {
   __shared__ SurvCursor s_cur[WAVES]; /* per wave, in LDS to spare registers */
   SurvCursor &sc = s_cur[threadIdx.x >> 6];
   sc = SurvCursor{0, 0};
-  RotConsts rc;
+  RotConstsT<rot_row_bytes(R0)> rc;
   if constexpr (R0) rc.init(threadIdx.x & 63);
   while (true) {
     __syncthreads();
@@ -1316,13 +1337,14 @@ sieve_queue2_kernel(SieveArgs a) {
   constexpr int Q0 = R0 ? 192 : 128;
   static_assert(qsieve2_lds_bytes<NL, WAVES, LV, R0, Q0, GQ>() + 16 * WAVES <= 160 * 1024, "LDS budget");
   static_assert(LV::B[1] <= NL, "level 0 must be in LDS");
-  static_assert(R0 == 0 || (R0 == ROT_N && V == 0 && G == 0), "rotation group replaces V/G level-0 forms");
+  static_assert(R0 == 0 || ((R0 == ROT_N || R0 == ROT_WIDE) && V == 0 && G == 0),
+                "rotation group replaces V/G level-0 forms");
   __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ> L;
   __shared__ SurvCursor s_cur[WAVES];
   stage_pair_tables<R0, NL>(L.tab, a.tab);
   SurvCursor &sc = s_cur[threadIdx.x >> 6];
   sc = SurvCursor{0, 0};
-  RotConsts rc;
+  RotConstsT<rot_row_bytes(R0)> rc;
   if constexpr (R0) rc.init(threadIdx.x & 63);
   __syncthreads();
   QSieve2<NL, LV, R0, Q0, TU, GQ> qs;

@@ -481,7 +481,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
-    ap.add_argument("--full-tune", action="store_true", help="autotune over all 33 candidates (11 variants x 3 work-item sizes)")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 36 candidates (12 variants x 3 work-item sizes)")
     ap.add_argument("--serve-loads", default="0.25,0.5,1.0",
                     help="offered loads of the serving sweep, as fractions of the saturated serving K/s")
     ap.add_argument("--serve-units", type=int, default=10)
