@@ -40,8 +40,10 @@ Beside it on the line:
                 in the mix).  Unit length floor(exp(U(0, ln 16))): the
                 workload's 1..8192 scaled down so a sweep finishes in the bench;
   cpu_baseline  the reference CPU build's own AVX2 search, compiled from its
-                sources (oracle/_ref), one process per host core the box
-                grants, each on ~20 s of a K of the plain workload; plus the
+                sources (oracle/_ref), one process per host core available
+                to this process (at most 16), each on ~20 s of a K of the
+                plain workload, half in the first and half in the last shift
+                group; plus the
                 oracle's SPRP on a sample of the micro-benchmark inputs,
                 checked against the GPU's outputs.
 """
@@ -431,7 +433,7 @@ def reference_cpu_sample(procs, seconds=20.0):
 
 
 def cpu_baseline(n_prp, gpu_prp, j1=2048):
-    """The CPU baseline on the GPU box's host cores: the reference CPU build
+    """The CPU baseline on this machine's host cores: the reference CPU build
     itself (kind "reference") when oracle/_ref holds its sampler, else the
     oracle's scalar restatement (kind "port") on a slice of one K; plus the
     oracle's SPRP on 2^16 micro-benchmark inputs, checked against the GPU's
