@@ -204,6 +204,7 @@ struct BatchState {
   int chunk = 0, nchunk = 0;
   int pvariant = -1;
   PersistVariant pv{};
+  int walk_depth = 1;        /* compaction depth of the batched walk */
   int64_t surv_cap = 0;      /* survivor slots per round */
   int64_t surv_slack = 0;    /* slots the sieve's waves may reserve and leave unused */
   int max_jobs = 0;          /* capacity of each io's d_jobs */
@@ -269,6 +270,7 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, int nio, BatchState **o
   e = getenv("AP26_BATCH_VARIANT");
   b->pvariant = e ? atoi(e) : (ctx->tune_variant >= 0 ? ctx->tune_variant : 0);
   b->pv = pick_pvariant(b->pvariant);
+  b->walk_depth = walk_depth_of(ctx);
   b->surv_slack = (int64_t)ctx->cu_count * b->pv.waves * SURV_CHUNK;
   b->surv_cap = (int64_t)b->per_round * (INT64_C(48) << 20) + b->surv_slack; /* ~40M survivors per K */
   /* guarded (i31, i37, i41) triples in the reference order */
@@ -410,7 +412,8 @@ static int enqueue_round(ap26_ctx *ctx, BatchState *b, int part, const RoundIO &
   w.nlive = o.d_cnt + 5;
   w.err = (unsigned int *)(o.d_cnt + 6);
   w.big = o.d_big;
-  hipLaunchKernelGGL(batch_walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+  const WalkFn<BatchWalkArgs> wfn = pick_batch_walk(b->walk_depth);
+  hipLaunchKernelGGL(wfn, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
   hipLaunchKernelGGL(round_advance_kernel, dim3(1), dim3(64), 0, ctx->stream, o.d_rs, o.d_cnt + 0,
                      (unsigned int *)(o.d_cnt + 3));
   return hipGetLastError() == hipSuccess ? AP26_OK : AP26_E_DEVICE;
@@ -1106,50 +1109,80 @@ extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t qu
   HIPCHK(hipSetDevice(ctx->device));
   const int nv = quick ? 5 : (int)(sizeof(TUNE_VARIANTS) / sizeof(int));
   const int nc = quick ? 2 : (int)(sizeof(TUNE_CHUNKS) / sizeof(int));
-  const int save_v = ctx->tune_variant, save_c = ctx->tune_chunk;
+  const int nd = (int)(sizeof(WALK_DEPTHS) / sizeof(int));
+  const int save_v = ctx->tune_variant, save_c = ctx->tune_chunk, save_d = ctx->tune_walk_depth;
   BatchState *save_b = ctx->batch;
-  ap26_tune_entry top{-1, -1, 1e30};
+  ap26_tune_entry top{-1, -1, 1, 0, 1e30, 1e30};
   int n = 0, rc = AP26_OK;
   ap26_range r{K, K, SHIFT, 0};
   ap26_sol *sols = (ap26_sol *)malloc(sizeof(ap26_sol) * 4096);
   if (!sols) return AP26_E_NOMEM;
+  /* one candidate: a fresh batched state built with the context's choice,
+     one warm call (graph capture), one timed call */
+  auto time_one = [&](ap26_tune_entry &t) {
+    BatchState *tmp = nullptr;
+    int e = batch_state_new(ctx, 1, 1, &tmp);
+    if (e) return e;
+    ctx->batch = tmp;
+    int32_t ns = 0;
+    int64_t tot = 0;
+    ap26_stats st{};
+    e = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st);
+    if (!e) e = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st);
+    ctx->batch = save_b;
+    batch_state_free(tmp);
+    t = ap26_tune_entry{ctx->tune_variant, ctx->tune_chunk, walk_depth_of(ctx), 0, st.ms_walk,
+                        st.ms_walk - st.ms_sieve};
+    return e;
+  };
+  /* stage 1: sieve variant x work-item size, at walk depth 1 */
+  ctx->tune_walk_depth = 1;
   for (int vi = 0; vi < nv && !rc; vi++)
     for (int ci = 0; ci < nc && !rc; ci++) {
       ctx->tune_variant = TUNE_VARIANTS[vi];
       ctx->tune_chunk = quick ? TUNE_CHUNKS[ci + 1] : TUNE_CHUNKS[ci];
-      BatchState *tmp = nullptr;
-      rc = batch_state_new(ctx, 1, 1, &tmp);
-      if (rc) break;
-      ctx->batch = tmp;
-      int32_t ns = 0;
-      int64_t tot = 0;
-      ap26_stats st{};
-      rc = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st); /* warm: builds the graphs */
-      if (!rc) rc = ap26_search_batch(ctx, &r, 1, sols, 4096, &ns, &tot, &st);
-      ctx->batch = save_b;
-      batch_state_free(tmp);
-      if (rc) break;
-      ap26_tune_entry t{ctx->tune_variant, ctx->tune_chunk, st.ms_walk};
+      ap26_tune_entry t{};
+      if ((rc = time_one(t))) break;
       if (results && n < cap) results[n] = t;
       n++;
       if (t.ms_per_k < top.ms_per_k) top = t;
     }
+  /* stage 2: the walk's compaction depth with that sieve, by the walk's own
+     time (the device time after the sieve graph) */
+  if (!rc) {
+    ctx->tune_variant = top.variant;
+    ctx->tune_chunk = top.chunk;
+    double best_walk = 1e30;
+    for (int di = 0; di < nd && !rc; di++) {
+      ctx->tune_walk_depth = WALK_DEPTHS[di];
+      ap26_tune_entry t{};
+      if ((rc = time_one(t))) break;
+      if (results && n < cap) results[n] = t;
+      n++;
+      if (t.ms_walk < best_walk) {
+        best_walk = t.ms_walk;
+        top = t;
+      }
+    }
+  }
   free(sols);
   ctx->batch = save_b;
   if (rc) {
     ctx->tune_variant = save_v;
     ctx->tune_chunk = save_c;
+    ctx->tune_walk_depth = save_d;
     return rc;
   }
   ctx->tune_variant = top.variant;
   ctx->tune_chunk = top.chunk;
+  ctx->tune_walk_depth = top.walk_depth;
   batch_free(ctx); /* the next batch is built with the tuned choice */
   if (n_out) *n_out = n;
   if (best) *best = top;
   if (cache_path && cache_path[0]) {
     FILE *f = fopen(cache_path, "w");
     if (!f) return AP26_E_RANGE;
-    fprintf(f, "ap26-tune 1 %s %d %d %.3f\n", ctx->name, top.variant, top.chunk, top.ms_per_k);
+    fprintf(f, "ap26-tune 2 %s %d %d %d %.3f\n", ctx->name, top.variant, top.chunk, top.walk_depth, top.ms_per_k);
     fclose(f);
   }
   return AP26_OK;
@@ -1162,12 +1195,15 @@ void tune_load_cache(ap26_ctx *ctx) {
   FILE *f = fopen(path, "r");
   if (!f) return;
   char arch[256];
-  int ver = 0, v = -1, c = -1;
+  int ver = 0, v = -1, c = -1, d = 1;
   double ms = 0;
-  if (fscanf(f, "ap26-tune %d %255s %d %d %lf", &ver, arch, &v, &c, &ms) == 5 && ver == 1 &&
-      strcmp(arch, ctx->name) == 0 && c > 0 && c <= AP26_NN43) {
+  /* version 1: variant, chunk; version 2 adds the walk depth */
+  if (fscanf(f, "ap26-tune %d %255s %d %d", &ver, arch, &v, &c) == 4 && (ver == 1 || ver == 2) &&
+      (ver == 1 || fscanf(f, "%d", &d) == 1) && fscanf(f, "%lf", &ms) == 1 && strcmp(arch, ctx->name) == 0 &&
+      c > 0 && c <= AP26_NN43) {
     ctx->tune_variant = v;
     ctx->tune_chunk = c;
+    ctx->tune_walk_depth = d;
   }
   fclose(f);
 }

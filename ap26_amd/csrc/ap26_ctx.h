@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 
 #include "../../include/ap26_gpu.h"
 #include "ap26_kernels.h"
@@ -89,12 +90,34 @@ struct ap26_ctx {
   BatchState *batch = nullptr;
   int tune_variant = -1; /* batched sieve variant / work-item chunk chosen by */
   int tune_chunk = -1;   /* ap26_autotune (or its cache); -1 = default       */
+  int tune_walk_depth = -1; /* walk compaction depth chosen the same way */
   void *scratch = nullptr; /* this device's address of the allocation-free entry
                               point's scratch array (hipGetSymbolAddress) */
 };
 
 
 SieveVariant pick_variant(int id);
+
+/* Walk compaction depth: AP26_WALK_DEPTH, else the autotuned choice, else 1;
+   one of WALK_DEPTHS. */
+static inline int walk_depth_of(const ap26_ctx *ctx) {
+  const char *e = getenv("AP26_WALK_DEPTH");
+  const int d = e ? atoi(e) : ctx->tune_walk_depth;
+  for (int w : WALK_DEPTHS)
+    if (d == w) return d;
+  return 1;
+}
+
+template <class Args>
+using WalkFn = void (*)(Args);
+
+/* the walk kernel of a depth */
+static inline WalkFn<WalkArgs> pick_walk(int depth) {
+  return depth == 4 ? &walk_kernel<4> : depth == 2 ? &walk_kernel<2> : &walk_kernel<1>;
+}
+static inline WalkFn<BatchWalkArgs> pick_batch_walk(int depth) {
+  return depth == 4 ? &batch_walk_kernel<4> : depth == 2 ? &batch_walk_kernel<2> : &batch_walk_kernel<1>;
+}
 
 /* batched engine state (ap26_batch.hip) */
 struct BatchState;

@@ -371,7 +371,8 @@ int ap26_search_span(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it0, int32
     w.err = (unsigned int *)(ctx->d_cnt + C_ERR);
     w.nlive = ctx->d_cnt + C_LIVE;
     w.big = ctx->d_cnt + C_BIG;
-    hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+    const WalkFn<WalkArgs> wfn = pick_walk(walk_depth_of(ctx));
+    hipLaunchKernelGGL(wfn, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
     hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, ctx->d_sols, w.nsol, SOL_CAP);
@@ -488,7 +489,8 @@ int ap26_search_span_noalloc(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t it
   w.err = (unsigned int *)(bf.cnt + C_ERR);
   w.nlive = bf.cnt + C_LIVE;
   w.big = bf.cnt + C_BIG;
-  hipLaunchKernelGGL(walk_kernel, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
+  const WalkFn<WalkArgs> wfn = pick_walk(walk_depth_of(ctx));
+  hipLaunchKernelGGL(wfn, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, w);
   hipLaunchKernelGGL(verify_kernel, dim3(256), dim3(256), 0, ctx->stream, bf.sols, w.nsol, bf.sol_cap);
   hipLaunchKernelGGL(verify_finish_kernel, dim3(16), dim3(256), 0, ctx->stream, bf.sols, w.nsol, bf.sol_cap);
   HIPCHK(hipGetLastError());

@@ -1387,7 +1387,13 @@ constexpr int WALK_QCAP = 128;
    the backward extension tests terms below the run's first forward term. */
 constexpr uint64_t WALK_LIMIT = UINT64_C(1) << 62;
 
-template <class StepOf, class Close, class Big>
+/* DEPTH: terms a lane tests in a row before a run that goes on is queued
+   again (the compaction depth).  1 keeps every pass dense; larger depths
+   trade idle lanes (runs that stop early in the pass) for fewer queue
+   round trips.  The terms tested, and so every result, are the same. */
+constexpr int WALK_DEPTHS[] = {1, 2, 4};
+
+template <class StepOf, class Close, class Big, int DEPTH = 1>
 struct WalkQueue {
   WEntry *q; /* this wave's WALK_QCAP entries */
   int qn = 0;
@@ -1407,19 +1413,22 @@ struct WalkQueue {
     __builtin_amdgcn_wave_barrier();
   }
 
-  /* test one term of each lane's run (pred lanes), requeue runs that go on */
+  /* test up to DEPTH terms of each lane's run (pred lanes), requeue runs
+     that go on */
   __device__ __forceinline__ void step(WEntry e, bool pred) {
-    bool go = false;
-    if (pred) {
-      tests++;
-      if ((uint64_t)e.m >= WALK_LIMIT) big(e.job);
-      go = sprp2((uint64_t)e.m);
+    bool go = pred;
+    for (int d = 0; d < DEPTH; d++) {
       if (go) {
-        e.k++;
-        e.m += step_of(e.job);
+        tests++;
+        if ((uint64_t)e.m >= WALK_LIMIT) big(e.job);
+        go = sprp2((uint64_t)e.m);
+        if (go) {
+          e.k++;
+          e.m += step_of(e.job);
+        }
       }
     }
-    push(e, pred && go);
+    push(e, go);
     if (pred && !go && e.k >= 10) close(e, tests);
   }
 
@@ -1478,6 +1487,7 @@ struct BatchWalkArgs {
   unsigned long long *__restrict__ big;      /* per range: walked terms >= 2^62 */
 };
 
+template <int DEPTH>
 static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a) {
   __shared__ WEntry s_q[4][WALK_QCAP];
   const RoundState rs = *a.rs;
@@ -1508,7 +1518,7 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
     }
   };
   auto big = [&](int32_t job) { atomicAdd(&a.big[jobs[job].range_id], 1ull); };
-  WalkQueue<decltype(step_of), decltype(close), decltype(big)> wq{
+  WalkQueue<decltype(step_of), decltype(close), decltype(big), DEPTH> wq{
       s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
   unsigned long long live = 0;
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -1563,6 +1573,7 @@ struct WalkArgs {
   unsigned long long *__restrict__ big;   /* walked terms >= 2^62 */
 };
 
+template <int DEPTH>
 static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   __shared__ WEntry s_q[4][WALK_QCAP];
   unsigned long long n = *a.nsurv;
@@ -1588,7 +1599,7 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   };
   unsigned long long nbig = 0;
   auto big = [&](int32_t) { nbig++; };
-  WalkQueue<decltype(step_of), decltype(close), decltype(big)> wq{
+  WalkQueue<decltype(step_of), decltype(close), decltype(big), DEPTH> wq{
       s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;

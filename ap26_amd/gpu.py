@@ -46,7 +46,12 @@ class Opts(ctypes.Structure):
 
 
 class TuneEntry(ctypes.Structure):
-    _fields_ = [("variant", ctypes.c_int32), ("chunk", ctypes.c_int32), ("ms_per_k", ctypes.c_double)]
+    _fields_ = [("variant", ctypes.c_int32), ("chunk", ctypes.c_int32), ("walk_depth", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("ms_per_k", ctypes.c_double), ("ms_walk", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"variant": self.variant, "chunk": self.chunk, "walk_depth": self.walk_depth,
+                "ms_per_k": self.ms_per_k, "ms_walk": self.ms_walk}
 
 
 class Stats(ctypes.Structure):
@@ -263,15 +268,15 @@ class Context:
         return out[:n.value].copy(), tot, st
 
     def autotune(self, K=48620637, shift=640, quick=True, cache_path=None):
-        """Pick the batched sieve variant / work-item size by measurement."""
+        """Pick the batched sieve variant / work-item size, then the walk's
+        compaction depth, by measurement (ap26_autotune)."""
         res = (TuneEntry * 64)()
         n = ctypes.c_int32()
         best = TuneEntry()
         _check(self._lib.ap26_autotune(self.h, K, shift, 1 if quick else 0,
                                        cache_path.encode() if cache_path else None, res, 64,
                                        ctypes.byref(n), ctypes.byref(best)), "ap26_autotune")
-        return ({"variant": best.variant, "chunk": best.chunk, "ms_per_k": best.ms_per_k},
-                [{"variant": r.variant, "chunk": r.chunk, "ms_per_k": r.ms_per_k} for r in res[:n.value]])
+        return best.as_dict(), [r.as_dict() for r in res[:n.value]]
 
     # device buffers
     def dev_alloc(self, nbytes):

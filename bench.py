@@ -19,8 +19,9 @@ timed region.  After timing, every unit's SOL records and hash line are
 compared with the reference CPU build's output for the same K
 (tests/golden/ref_min10); a mismatch fails the run (exit status 1).
 
-The batched engine's sieve variant and work-item size are chosen first by the
-library's autotuner (ap26_autotune, 10 candidates timed on one K).
+The batched engine's sieve variant and work-item size, then its walk's
+compaction depth, are chosen first by the library's autotuner (ap26_autotune,
+10 + 3 candidates timed on one K).
 
 Beside it on the line:
   plain         the plain per-K path (one kernel sequence per K, host sync per
@@ -483,7 +484,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
-    ap.add_argument("--full-tune", action="store_true", help="autotune over all 36 candidates (12 variants x 3 work-item sizes)")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 39 candidates (12 variants x 3 work-item sizes, then 3 walk depths)")
     ap.add_argument("--serve-loads", default="0.25,0.5,1.0",
                     help="offered loads of the serving sweep, as fractions of the saturated serving K/s")
     ap.add_argument("--serve-units", type=int, default=10)
@@ -499,11 +500,13 @@ def main():
     ctx = gpu.Context(0)
     if args.no_tune:
         tune_best = {"variant": int(os.environ.get("AP26_BATCH_VARIANT", "0")),
-                     "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "85"))}
+                     "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "85")),
+                     "walk_depth": int(os.environ.get("AP26_WALK_DEPTH", "1"))}
         tune_all = []
     else:
         tune_best, tune_all = ctx.autotune(K=W.C3_K0 + 2, shift=W.C3_SHIFT, quick=not args.full_tune)
-    progress(f"autotuned: variant {tune_best['variant']}, chunk {tune_best['chunk']}")
+    progress(f"autotuned: variant {tune_best['variant']}, chunk {tune_best['chunk']}, "
+             f"walk depth {tune_best['walk_depth']}")
     b = run_batched(ctx, args.steps, args.warmup)
     kps = b["k"] / b["seconds"]
     progress(f"batched: {b['k']} K in {b['seconds']:.2f} s, parity {b['parity']['verified']}/{b['units']} units")
@@ -538,7 +541,7 @@ def main():
                    "k_searched": b["k"], "units": b["units"], "unit_index": b["unit_index"], "shifts": b["shifts"],
                    "units_across_rounds": b["units_across_rounds"], "rounds": rounds, "path": "batched",
                    "autotuned": {"variant": tune_best["variant"], "chunk": tune_best["chunk"],
-                                 "candidates": len(tune_all)}},
+                                 "walk_depth": tune_best["walk_depth"], "candidates": len(tune_all)}},
         "parity": b["parity"],
         "roofline": {"bound": "valu_int32", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s",
                      "frac": achieved / PEAK_INT32_TOPS if achieved else None,

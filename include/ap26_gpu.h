@@ -204,15 +204,22 @@ int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k_done, int6
                       int32_t *k_per_round, int32_t *active);
 
 /* ---- autotuner --------------------------------------------------------
-   Times the batched engine's sieve variants (LDS-staged primes, wavefronts
-   per CU, compaction levels, one shift or a shift pair per work item) and
-   work-item sizes on one K at SHIFT, keeps the fastest for this ctx, and
-   writes it to cache_path (if not NULL); ap26_open adopts a cache named by the
-   AP26_TUNE_CACHE environment variable.  quick = 1 tries 10 candidates. */
+   Times the batched engine on one K at SHIFT: first its sieve variants
+   (LDS-staged primes, wavefronts per CU, compaction levels, one shift or a
+   shift pair per work item) and work-item sizes, keeping the fastest by
+   device time; then, with that sieve, the walk's compaction depth (terms a
+   lane tests before a run is queued again: 1, 2 or 4), keeping the fastest
+   by the walk's time.  The choice holds for this ctx and is written to
+   cache_path (if not NULL); ap26_open adopts a cache named by the
+   AP26_TUNE_CACHE environment variable.  quick = 1 tries 10 + 3 candidates,
+   results[] lists every candidate timed in that order. */
 typedef struct {
   int32_t variant;
-  int32_t chunk;     /* n43 starts per work item */
-  double ms_per_k;   /* device time of the sample K */
+  int32_t chunk;      /* n43 starts per work item */
+  int32_t walk_depth; /* walk compaction depth */
+  int32_t reserved;
+  double ms_per_k;    /* device time of the sample K */
+  double ms_walk;     /* of which after the sieve (walk and verification) */
 } ap26_tune_entry;
 
 int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -250,20 +250,52 @@ def test_server_admission_limit(gpu_ctx):
 
 
 def test_autotune_picks_and_caches(tmp_path, monkeypatch):
+    """Stage 1 picks the sieve (variant, chunk) by device time at walk depth
+    1; stage 2 times the walk depths 1, 2, 4 with that sieve and picks by the
+    walk's time; the cache carries all three and a new context adopts it."""
+    monkeypatch.delenv("AP26_WALK_DEPTH", raising=False)
     cache = tmp_path / "tune.txt"
     with gpu.Context(0) as ctx:
         best, tried = ctx.autotune(K=875859, shift=640, quick=True, cache_path=str(cache))
-        assert len(tried) == 10
-        assert best["ms_per_k"] == min(t["ms_per_k"] for t in tried)
+        assert len(tried) == 13
+        sieve, walk = tried[:10], tried[10:]
+        assert all(t["walk_depth"] == 1 for t in sieve)
+        first = min(sieve, key=lambda t: t["ms_per_k"])
+        assert (best["variant"], best["chunk"]) == (first["variant"], first["chunk"])
+        assert [t["walk_depth"] for t in walk] == [1, 2, 4]
+        assert all((t["variant"], t["chunk"]) == (best["variant"], best["chunk"]) for t in walk)
+        assert best["ms_walk"] == min(t["ms_walk"] for t in walk)
+        assert 0 < best["ms_walk"] < best["ms_per_k"]
         want = plain_reports(ctx, 875859, 875859, 640)
         sols, tot, st = ctx.search_batch([(875859, 875859, 640, 0)])  # built with the choice
         assert key(sols) == want
     fields = cache.read_text().split()
-    assert fields[0] == "ap26-tune" and int(fields[3]) == best["variant"] and int(fields[4]) == best["chunk"]
+    assert fields[:2] == ["ap26-tune", "2"]
+    assert [int(x) for x in fields[3:6]] == [best["variant"], best["chunk"], best["walk_depth"]]
     monkeypatch.setenv("AP26_TUNE_CACHE", str(cache))
     with gpu.Context(0) as ctx:  # adopts the cached choice
         sols, tot, st = ctx.search_batch([(875859, 875859, 640, 0)])
         assert key(sols) == want
+
+
+@pytest.mark.parametrize("depth", ["1", "2", "4"])
+def test_walk_depth_keeps_results(depth, monkeypatch):
+    """Every walk compaction depth tests the same terms: the batched engine
+    (two rounds, mixed SHIFTs, resumed shifts) and the plain path give the
+    reports, hash totals and SPRP test counts of depth 1."""
+    ranges = RANGES + [(48620635, 48620654, 640, 0)]
+    monkeypatch.setenv("AP26_WALK_DEPTH", "1")
+    with gpu.Context(0) as ctx:
+        base = ctx.search_batch(ranges)
+        base_plain = ctx.search_k(366384, 0)
+    monkeypatch.setenv("AP26_WALK_DEPTH", depth)
+    with gpu.Context(0) as ctx:
+        sols, tot, st = ctx.search_batch(ranges)
+        psols, pst = ctx.search_k(366384, 0)
+    assert key(sols) == key(base[0])
+    assert [int(x) for x in tot] == [int(x) for x in base[1]]
+    assert st.prp_tests == base[2].prp_tests and st.survivors == base[2].survivors
+    assert key(psols) == key(base_plain[0]) and pst.prp_tests == base_plain[1].prp_tests
 
 
 def test_batch_equals_plain_on_mixed_workload_sample(gpu_ctx):
