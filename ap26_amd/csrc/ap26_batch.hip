@@ -77,10 +77,11 @@ PersistVariant make_pvariant2rg() {
    the wave priorities and how many rotated gathers are in flight at once
    (Tune).  75..91 move the deeper queues to global memory and spend the
    LDS on more table primes: a level that reads its tables through L1/L2
-   costs more than the queue traffic (0 = 85, the default: the last queue
-   in global memory, primes 61..251 in LDS).  92..102 use the 512-byte
-   rotation rows (ROT_WIDE: no per-lane column registers or adds, twice the
-   LDS); 16 waves still spill the rotation constants in the hot loop. */
+   costs more than the queue traffic (85: the last queue in global memory,
+   primes 61..251 in LDS).  92..102 use the 512-byte rotation rows
+   (ROT_WIDE: no per-lane column registers or adds, twice the LDS; 0 = 95,
+   the default: queues 1 and 2 in global memory, primes 61..233 in LDS);
+   16 waves still spill the rotation constants in the hot loop. */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -176,7 +177,7 @@ PersistVariant pick_pvariant(int id) {
     case 101: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 1>();
     case 102: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
-    default: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>(); /* = 85 */
+    default: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 95 */
   }
 }
 
@@ -1100,7 +1101,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {85, 86, 90, 88, 70, 95, 75, 67, 58, 54, 28, 10};
+static const int TUNE_VARIANTS[] = {95, 85, 98, 101, 100, 86, 90, 88, 70, 75, 54, 28};
 static const int TUNE_CHUNKS[] = {43, 85, 170};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

@@ -66,6 +66,13 @@ SieveVariant make_q2variantrg() {
                       qsieve2_global_entries<LV, GQ>()};
 }
 
+/* the same with 512-byte rotation rows (no per-lane column registers) */
+template <int NL, int WAVES, class LV, class TU, int GQ>
+SieveVariant make_q2variantrw() {
+  return SieveVariant{2, NL, 64 * WAVES, &sieve_queue2_kernel<NL, WAVES, LV, 0, 0, ROT_WIDE, TU, GQ>,
+                      qsieve2_global_entries<LV, GQ>()};
+}
+
 
 struct BatchState;
 struct ap26_ctx {

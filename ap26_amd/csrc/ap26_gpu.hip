@@ -20,13 +20,15 @@ using namespace ap26;
 
 #include "ap26_ctx.h"
 
-/* Sieve variants: 0 = default (45: shift-pair queued sieve with the
-   bank-rotated level-0 group, level 0 on 20 primes, 12 waves per CU), 1 =
-   simple S=1, 2 = simple S=2, 10..30 = one-shift queued-sieve tuning
-   candidates (LDS primes, waves per CU, levels, VGPR constants), 31..33 =
-   shift-pair queued sieves without the rotation group, 34..36 = shift-pair
-   with the first 1..3 level-0 primes from register bit windows instead of
-   LDS, 40..45 = rotation-group forms. */
+/* Sieve variants: 0 = default (47: shift-pair queued sieve with the
+   bank-rotated level-0 group in 512-byte rows, level 0 on 20 primes, 12
+   waves per CU, queues 1 and 2 in global memory), 1 = simple S=1, 2 =
+   simple S=2, 10..30 = one-shift queued-sieve tuning candidates (LDS
+   primes, waves per CU, levels, VGPR constants), 31..33 = shift-pair queued
+   sieves without the rotation group, 34..36 = shift-pair with the first
+   1..3 level-0 primes from register bit windows instead of LDS, 40..47 =
+   rotation-group forms (45: every queue in LDS, the allocation-free entry
+   point's sieve). */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -66,7 +68,9 @@ SieveVariant pick_variant(int id) {
     case 45: return make_q2variantr<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>();
     /* 46: the last queue in global memory, 37 primes in LDS */
     case 46: return make_q2variantrg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    default: return make_q2variantrg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>(); /* = 46 */
+    /* 47: 512-byte rotation rows, queues 1.. in global memory, 34 primes in LDS */
+    case 47: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    default: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 47 */
   }
 }
 

@@ -876,10 +876,12 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
 
 /* Level 0 of a fresh position with the rotation group: both masks of the
    pair over primes 0..E-1, the first ROT_N through the rotated rows, the rest
-   (ROT_N..E-1) from the packed LDS tables. */
+   (ROT_N..E-1) from the packed LDS tables.  init (all ones, or zero for a
+   lane whose position is only a stand-in) starts the AND chains, so such a
+   lane's masks come out zero at no extra cost. */
 template <int E, int SPLIT = 2, int W = 256>
-__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConstsT<W> &rc, uint64_t &m0,
-                                           uint64_t &m1) {
+__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConstsT<W> &rc, uint32_t init,
+                                           uint64_t &m0, uint64_t &m1) {
   static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
   static_assert(W == 256 || W == 512, "row width");
   constexpr int PART = ROT_N / SPLIT;
@@ -889,7 +891,7 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
      wide rows: plus the row offset of residue 0 and this lane's column */
   const uint32_t xsw = lo32(xd) * (uint32_t)W + (W == 512 ? 512u * ROT_HALF + 16u * (threadIdx.x & 15) : 0u);
   const uint32_t xs16 = lo32(xd) * 16u;
-  uint32_t a0 = ~0u, a1 = ~0u, a2 = ~0u, a3 = ~0u;
+  uint32_t a0 = init, a1 = init, a2 = init, a3 = init;
   /* SPLIT parts of the group, so that at most 16/SPLIT gathers are in flight */
   static_for<0, SPLIT>([&](auto HC) {
     constexpr int H = decltype(HC)::value;
@@ -1117,7 +1119,9 @@ struct QSieve2 {
   QEntry *qb;  /* this wave's LDS queues: queue 0 (Q0 entries), then 128 per level below GQ */
   QEntry *gqb; /* this wave's global queues GQ.. (128 entries each) */
   int qn[NQ];
-  bool partial; /* a queue step ran with lanes switched off (must never happen) */
+  uint64_t seen_exec; /* AND of the exec masks of every queue step: not all ones if
+                         a step ran with lanes switched off (must never happen);
+                         wave-uniform, so it stays in scalar registers */
   const char *ldsb, *gtb;
   int lane;
   const SieveArgs *a;
@@ -1129,7 +1133,7 @@ struct QSieve2 {
      once that the whole wave runs it, and a failure is reported (the queue
      would lose entries). */
   __device__ __forceinline__ void check_full_wave() {
-    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
+    seen_exec &= __builtin_amdgcn_read_exec();
   }
 
   /* A wave reads back its own global queue entries, written by other lanes
@@ -1194,7 +1198,7 @@ struct QSieve2 {
       }
       flush<L + 1>();
     } else {
-      if (partial && lane == 0) atomicOr(a->err, 1u);
+      if (seen_exec != ~UINT64_C(0) && lane == 0) atomicOr(a->err, 1u);
     }
   }
 };
@@ -1213,17 +1217,22 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
     const bool has = b < a.blk1;
     if (__ballot(has) == 0) break;
     double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
+    uint32_t init = has ? ~0u : 0u; /* lanes past the last block: masks of zero */
+    /* opaque to the optimiser, which would otherwise turn the AND with init
+       back into a select per mask word after the chains */
+    asm volatile("" : "+v"(init));
     for (int i59 = 0; i59 < AP26_N59_STEPS; i59++) {
       qs.check_full_wave();
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(TU::prio0);
-      if constexpr (R0)
-        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0)>(xd, qs.ldsb, *rc, m0, m1);
-      else
+      if constexpr (R0) {
+        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0)>(xd, qs.ldsb, *rc, init, m0, m1);
+      } else {
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
+        m0 = has ? m0 : 0;
+        m1 = has ? m1 : 0;
+      }
       __builtin_amdgcn_s_setprio(0);
-      m0 = has ? m0 : 0;
-      m1 = has ? m1 : 0;
       qs.template push<0>(xd, m0, m0 != 0);
       if constexpr (Q0 == 128) qs.template drain<0>();
       qs.template push<0>(xd + TWO48, m1, m1 != 0);
@@ -1318,7 +1327,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     qs.gqb = p.gq + ((size_t)blockIdx.x * WAVES + wave) * qsieve2_global_entries<LV, GQ>();
     for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU, GQ>::NQ; i++) qs.qn[i] = 0;
-    qs.partial = false;
+    qs.seen_exec = ~UINT64_C(0);
     qs.ldsb = (const char *)L.tab;
     qs.gtb = (const char *)tab;
     qs.a = &a;
@@ -1354,7 +1363,7 @@ sieve_queue2_kernel(SieveArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   qs.gqb = (QEntry *)a.gq + ((size_t)blockIdx.x * WAVES + wave) * qsieve2_global_entries<LV, GQ>();
   for (int i = 0; i < QSieve2<NL, LV, R0, Q0, TU, GQ>::NQ; i++) qs.qn[i] = 0;
-  qs.partial = false;
+  qs.seen_exec = ~UINT64_C(0);
   qs.ldsb = (const char *)L.tab;
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
@@ -1398,7 +1407,8 @@ struct WalkQueue {
   WEntry *q; /* this wave's WALK_QCAP entries */
   int qn = 0;
   int lane;
-  bool partial = false;
+  uint64_t seen_exec = ~UINT64_C(0); /* AND of the exec masks of every push (all ones
+                                        unless a push ran on part of the wave) */
   unsigned long long tests = 0;
   StepOf step_of;
   Close close;
@@ -1409,7 +1419,7 @@ struct WalkQueue {
     const int pre = __popcll(bal & ((UINT64_C(1) << lane) - 1));
     if (pred) q[qn + pre] = e;
     qn += __popcll(bal);
-    partial |= __builtin_amdgcn_read_exec() != ~UINT64_C(0);
+    seen_exec &= __builtin_amdgcn_read_exec();
     __builtin_amdgcn_wave_barrier();
   }
 
@@ -1519,7 +1529,7 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
   };
   auto big = [&](int32_t job) { atomicAdd(&a.big[jobs[job].range_id], 1ull); };
   WalkQueue<decltype(step_of), decltype(close), decltype(big), DEPTH> wq{
-      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
+      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), ~UINT64_C(0), 0, step_of, close, big};
   unsigned long long live = 0;
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
@@ -1539,7 +1549,7 @@ static __global__ void __launch_bounds__(256) batch_walk_kernel(BatchWalkArgs a)
     wq.drain();
   }
   wq.flush();
-  if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
+  if (wq.seen_exec != ~UINT64_C(0) && wq.lane == 0) atomicOr(a.err, 1u);
   if (wq.tests) atomicAdd(a.ntests, wq.tests);
   if (live) atomicAdd(a.nlive, live);
 }
@@ -1600,7 +1610,7 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
   unsigned long long nbig = 0;
   auto big = [&](int32_t) { nbig++; };
   WalkQueue<decltype(step_of), decltype(close), decltype(big), DEPTH> wq{
-      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), false, 0, step_of, close, big};
+      s_q[threadIdx.x >> 6], 0, (int)(threadIdx.x & 63), ~UINT64_C(0), 0, step_of, close, big};
   const unsigned long long wave = ((unsigned long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const unsigned long long nwaves = ((unsigned long long)gridDim.x * blockDim.x) >> 6;
   unsigned long long live = 0;
@@ -1618,7 +1628,7 @@ static __global__ void __launch_bounds__(256) walk_kernel(WalkArgs a) {
     wq.drain();
   }
   wq.flush();
-  if (wq.partial && wq.lane == 0) atomicOr(a.err, 1u);
+  if (wq.seen_exec != ~UINT64_C(0) && wq.lane == 0) atomicOr(a.err, 1u);
   if (wq.tests) atomicAdd(a.ntests, wq.tests);
   if (live) atomicAdd(a.nlive, live);
   if (nbig) atomicAdd(a.big, nbig);

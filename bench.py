@@ -97,16 +97,19 @@ PEAK_HBM_GBS = 8000.0
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
 # nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
 # sieve's load, so per-clock rates are about 6 % higher).
-SIEVE_VALU_LANE_INSTR_PER_K = 4.2603e12
+SIEVE_VALU_LANE_INSTR_PER_K = 3.8736e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 254,429 + 54,309,600) KiB / 16 K,
-# most of it the default kernel's global-memory queue written back from L2.
-TRAFFIC_BYTES_PER_K = 3.508e9
+# (profiles/sieve_rot_traffic.md): (2 x 155,709 + 26,357,417) KiB / 16 K,
+# most of it the default kernel's global-memory queues written back from L2.
+TRAFFIC_BYTES_PER_K = 1.7068e9
+# The batched engine's sieve when no variant is named (pick_pvariant(0) in
+# ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
+DEFAULT_BATCH_VARIANT = 95
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
 # n43 starts (10,840 x 8 B) read once.
@@ -520,7 +523,8 @@ def main():
     alg_bytes = (st["survivors"] * SURVIVOR_BYTES + b["k"] * TABLE_BYTES_PER_K) / rounds
     rate = SIEVE_VALU_LANE_INSTR_PER_K * k_per_launch / 64 / (avg_launch_ms / 1e3 * CLOCK_HZ * CUS) \
         if avg_launch_ms else None
-    kernel_name = f"sieve_persistent2_kernel (batched variant {tune_best['variant']}, chunk {tune_best['chunk']})"
+    variant = tune_best["variant"] or DEFAULT_BATCH_VARIANT
+    kernel_name = f"sieve_persistent2_kernel (batched variant {variant}, chunk {tune_best['chunk']})"
     line = {
         "metric": METRIC,
         "value": kps,
