@@ -745,10 +745,20 @@ constexpr int ROT_WIDE = 2 * ROT_N;
 constexpr int rot_primes(int R0) { return R0 ? ROT_N : 0; }            /* first packed prime */
 constexpr int rot_row_bytes(int R0) { return R0 == ROT_WIDE ? 512 : 256; }
 
-/* LDS bytes in front of the packed shift-pair tables. */
+/* LDS bytes of the rotation rows. */
 template <int R0>
 constexpr int rot_lds_bytes() {
   return R0 ? rot_row_bytes(R0) * ROT_ROWS : 0;
+}
+
+/* LDS bytes of the packed shift-pair tables of primes rot_primes(R0)..NL-1
+   and the 7..23 filters.  They come first in the table area, so that every
+   packed entry's offset fits the 16-bit immediate of the LDS read (no add
+   per lookup); the rotation rows follow, and their base is folded into the
+   per-position multiply-add addend of level 0. */
+template <int R0, int NL>
+constexpr int packed_lds_bytes() {
+  return 16 * (PT.off[NL] - PT.off[rot_primes(R0)] + 90);
 }
 
 template <int W>
@@ -773,16 +783,17 @@ struct RotConstsT {
 };
 using RotConsts = RotConstsT<256>;
 
-/* Stage one shift pair's tables into LDS: the rotation rows (R0 = ROT_N),
-   the packed entries of primes R0..NL-1 and the 7..23 filters.  tab is the
-   pair's global table, [entry][shift in pair], 16 bytes per entry. */
+/* Stage one shift pair's tables into LDS: the packed entries of primes
+   rot_primes(R0)..NL-1, the 7..23 filters, then the rotation rows (R0 != 0).
+   tab is the pair's global table, [entry][shift in pair], 16 bytes per
+   entry. */
 template <int R0, int NL>
 __device__ __forceinline__ void stage_pair_tables(char *lds, const uint64_t *__restrict__ tab) {
-  constexpr int RB = rot_lds_bytes<R0>();
+  constexpr int RB = packed_lds_bytes<R0, NL>();
   if constexpr (R0) {
     constexpr int NC = rot_row_bytes(R0) / 16; /* columns per row: 16 or 32 */
     const uint4 *t4 = (const uint4 *)tab;
-    uint4 *rows = (uint4 *)lds;
+    uint4 *rows = (uint4 *)(lds + RB);
     for (int i = threadIdx.x; i < ROT_ROWS * NC; i += blockDim.x) {
       const int q = i & (ROT_N - 1);
       const int p = PT.p[q];
@@ -791,7 +802,7 @@ __device__ __forceinline__ void stage_pair_tables(char *lds, const uint64_t *__r
       rows[i] = t4[PT.off[q] + c];
     }
   }
-  uint64_t *packed = (uint64_t *)(lds + RB);
+  uint64_t *packed = (uint64_t *)lds;
   constexpr int P0 = rot_primes(R0);
   constexpr int NPK = 2 * (PT.off[NL] - PT.off[P0]);
   for (int i = threadIdx.x; i < NPK; i += blockDim.x) packed[i] = tab[2 * PT.off[P0] + i];
@@ -846,7 +857,7 @@ __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const c
     static_for<B, E>([&](auto IC) {
       constexpr int I = decltype(IC)::value;
       if constexpr (I < NL) {
-        constexpr int C0 = rot_lds_bytes<R0>() + 16 * (PT.off[I] - PT.off[rot_primes(R0)] + (PT.p[I] - 1) / 2);
+        constexpr int C0 = 16 * (PT.off[I] - PT.off[rot_primes(R0)] + (PT.p[I] - 1) / 2);
         w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
       } else {
         constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
@@ -868,7 +879,7 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
   static_for<NSP, NPT>([&](auto IC) {
     constexpr int I = decltype(IC)::value;
     const int bo = centred_byteoff<PT.p[I], 16>(x, xs);
-    constexpr int C0 = rot_lds_bytes<R0>() + 16 * (LMAIN + PT.off[I] - NOK + (PT.p[I] - 1) / 2);
+    constexpr int C0 = 16 * (LMAIN + PT.off[I] - NOK + (PT.p[I] - 1) / 2);
     m &= *(const uint64_t *)(ldsb + C0 + bo);
   });
   return m;
@@ -878,8 +889,9 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
    pair over primes 0..E-1, the first ROT_N through the rotated rows, the rest
    (ROT_N..E-1) from the packed LDS tables.  init (all ones, or zero for a
    lane whose position is only a stand-in) starts the AND chains, so such a
-   lane's masks come out zero at no extra cost. */
-template <int E, int SPLIT = 2, int W = 256>
+   lane's masks come out zero at no extra cost.  RBASE: byte offset of the
+   rotation rows in the table area (after the packed tables). */
+template <int E, int SPLIT, int W, int RBASE>
 __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConstsT<W> &rc, uint32_t init,
                                            uint64_t &m0, uint64_t &m1) {
   static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
@@ -887,9 +899,11 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
   constexpr int PART = ROT_N / SPLIT;
   static_assert(E >= ROT_N, "level 0 holds the rotation group");
   const double x = xd - TWO52;
-  /* W * x mod 2^32 (exact: W * 2^24 = 0 mod 2^32 for the i24 multiply-add);
-     wide rows: plus the row offset of residue 0 and this lane's column */
-  const uint32_t xsw = lo32(xd) * (uint32_t)W + (W == 512 ? 512u * ROT_HALF + 16u * (threadIdx.x & 15) : 0u);
+  /* W * x mod 2^32 (exact: W * 2^24 = 0 mod 2^32 for the i24 multiply-add)
+     plus the offset of the rows' residue 0 (the rows' base and ROT_HALF
+     rows); wide rows: plus this lane's column */
+  const uint32_t xsw = lo32(xd) * (uint32_t)W + (uint32_t)(RBASE + W * ROT_HALF) +
+                       (W == 512 ? 16u * (threadIdx.x & 15) : 0u);
   const uint32_t xs16 = lo32(xd) * 16u;
   uint32_t a0 = init, a1 = init, a2 = init, a3 = init;
   /* SPLIT parts of the group, so that at most 16/SPLIT gathers are in flight */
@@ -904,7 +918,7 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
       if constexpr (W == 512)
         w[decltype(KC)::value] = *(const uint4 *)(ldsb + r + 16 * k); /* column (lane mod 16) + k */
       else
-        w[decltype(KC)::value] = *(const uint4 *)(ldsb + 256 * ROT_HALF + r + rc.col[k]);
+        w[decltype(KC)::value] = *(const uint4 *)(ldsb + r + rc.col[k]);
     });
     interleave_residues_and_reads<PART>();
     static_for<0, PART>([&](auto KC) {
@@ -920,7 +934,7 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
     static_for<ROT_N, E>([&](auto IC) {
       constexpr int I = decltype(IC)::value;
       const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
-      constexpr int C0 = W * ROT_ROWS + 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
+      constexpr int C0 = 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
       w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
     });
     static_for<0, E - ROT_N>([&](auto IC) {
@@ -1226,7 +1240,8 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(TU::prio0);
       if constexpr (R0) {
-        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0)>(xd, qs.ldsb, *rc, init, m0, m1);
+        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0), packed_lds_bytes<R0, NL>()>(xd, qs.ldsb, *rc, init, m0,
+                                                                                         m1);
       } else {
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
         m0 = has ? m0 : 0;
@@ -1246,8 +1261,8 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
 
 template <int NL, int WAVES, class LV, int R0 = 0, int Q0 = 128, int GQ = NO_GLOBAL_QUEUES>
 struct QSieve2Lds {
-  /* rotation rows (R0 = ROT_N), packed pair entries of primes R0..NL-1, filters */
-  alignas(16) char tab[rot_lds_bytes<R0>() + 16 * (PT.off[NL] - PT.off[rot_primes(R0)] + 90)];
+  /* packed pair entries of primes rot_primes(R0)..NL-1, filters, rotation rows */
+  alignas(16) char tab[packed_lds_bytes<R0, NL>() + rot_lds_bytes<R0>()];
   QEntry queues[WAVES][qsieve2_wave_entries<Q0, LV, GQ>()];
 };
 

@@ -97,16 +97,16 @@ PEAK_HBM_GBS = 8000.0
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
 # nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
 # sieve's load, so per-clock rates are about 6 % higher).
-SIEVE_VALU_LANE_INSTR_PER_K = 3.8736e12
+SIEVE_VALU_LANE_INSTR_PER_K = 3.6431e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 155,709 + 26,357,417) KiB / 16 K,
+# (profiles/sieve_rot_traffic.md): (2 x 168,827 + 34,949,812) KiB / 16 K,
 # most of it the default kernel's global-memory queues written back from L2.
-TRAFFIC_BYTES_PER_K = 1.7068e9
+TRAFFIC_BYTES_PER_K = 2.2584e9
 # The batched engine's sieve when no variant is named (pick_pvariant(0) in
 # ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
 DEFAULT_BATCH_VARIANT = 95
