@@ -24,6 +24,10 @@ using namespace ap26;
 namespace {
 
 constexpr int BATCH_SOL_CAP = 1 << 20;
+/* n43 starts per work item: small items keep the end of a round short (the
+   CUs finish their last items within a few ms of each other), and restaging
+   the LDS tables when a CU moves to another (K, shift pair) costs little */
+constexpr int DEFAULT_CHUNK = 11;
 constexpr int BATCH_SEG_K = 2048; /* searched K per segment of one batched call */
 constexpr int MAX_RANGES = 1 << 20;
 
@@ -265,8 +269,8 @@ static int batch_state_new(ap26_ctx *ctx, int per_round, int nio, BatchState **o
   b->nio = nio;
   /* environment overrides > autotuned choice > defaults */
   const char *e = getenv("AP26_BATCH_CHUNK");
-  b->chunk = e ? atoi(e) : (ctx->tune_chunk > 0 ? ctx->tune_chunk : 85);
-  if (b->chunk < 1 || b->chunk > AP26_NN43) b->chunk = 85;
+  b->chunk = e ? atoi(e) : (ctx->tune_chunk > 0 ? ctx->tune_chunk : DEFAULT_CHUNK);
+  if (b->chunk < 1 || b->chunk > AP26_NN43) b->chunk = DEFAULT_CHUNK;
   b->nchunk = (AP26_NN43 + b->chunk - 1) / b->chunk;
   e = getenv("AP26_BATCH_VARIANT");
   b->pvariant = e ? atoi(e) : (ctx->tune_variant >= 0 ? ctx->tune_variant : 0);
@@ -1102,7 +1106,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    AP26_TUNE_CACHE names it. */
 
 static const int TUNE_VARIANTS[] = {95, 85, 98, 101, 100, 86, 90, 88, 70, 75, 54, 28};
-static const int TUNE_CHUNKS[] = {43, 85, 170};
+static const int TUNE_CHUNKS[] = {11, 21, 43, 85};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,
                              ap26_tune_entry *results, int32_t cap, int32_t *n_out, ap26_tune_entry *best) {
@@ -1141,7 +1145,7 @@ extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t qu
   for (int vi = 0; vi < nv && !rc; vi++)
     for (int ci = 0; ci < nc && !rc; ci++) {
       ctx->tune_variant = TUNE_VARIANTS[vi];
-      ctx->tune_chunk = quick ? TUNE_CHUNKS[ci + 1] : TUNE_CHUNKS[ci];
+      ctx->tune_chunk = TUNE_CHUNKS[ci];
       ap26_tune_entry t{};
       if ((rc = time_one(t))) break;
       if (results && n < cap) results[n] = t;

@@ -487,12 +487,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-prp", action="store_true")
-    ap.add_argument("--full-tune", action="store_true", help="autotune over all 39 candidates (12 variants x 3 work-item sizes, then 3 walk depths)")
+    ap.add_argument("--full-tune", action="store_true", help="autotune over all 51 candidates (12 variants x 4 work-item sizes, then 3 walk depths)")
     ap.add_argument("--serve-loads", default="0.25,0.5,1.0",
                     help="offered loads of the serving sweep, as fractions of the saturated serving K/s")
     ap.add_argument("--serve-units", type=int, default=10)
     ap.add_argument("--no-tune", action="store_true",
-                    help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (0, 85)")
+                    help="skip the autotuner: AP26_BATCH_VARIANT / AP26_BATCH_CHUNK or the defaults (0, 11)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -503,7 +503,7 @@ def main():
     ctx = gpu.Context(0)
     if args.no_tune:
         tune_best = {"variant": int(os.environ.get("AP26_BATCH_VARIANT", "0")),
-                     "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "85")),
+                     "chunk": int(os.environ.get("AP26_BATCH_CHUNK", "11")),
                      "walk_depth": int(os.environ.get("AP26_WALK_DEPTH", "1"))}
         tune_all = []
     else:
