@@ -95,18 +95,19 @@ PEAK_HBM_GBS = 8000.0
 # 32 waves per CU; three-source ones and multiplies reach 0.82-0.92), so
 # rate / ceiling is the issue-slot use.  The nominal INT32
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
-# nominal 2.4 GHz (the counters show the clock near 2.27 GHz under the
-# sieve's load, so per-clock rates are about 6 % higher).
-SIEVE_VALU_LANE_INSTR_PER_K = 3.6431e12
+# nominal 2.4 GHz (the counters show the clock at 2.20-2.36 GHz under the
+# sieve's load, box to box, so per-clock rates are 2-9 % higher).
+SIEVE_VALU_LANE_INSTR_PER_K = 3.6452e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 168,827 + 34,949,812) KiB / 16 K,
-# most of it the default kernel's global-memory queues written back from L2.
-TRAFFIC_BYTES_PER_K = 2.2584e9
+# (profiles/sieve_rot_traffic.md): (2 x 163,498 + 13,863,955) KiB / 16 K,
+# the excess over the algorithmic bytes the global-memory queues written back
+# from L2.
+TRAFFIC_BYTES_PER_K = 9.082e8
 # The batched engine's sieve when no variant is named (pick_pvariant(0) in
 # ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
 DEFAULT_BATCH_VARIANT = 95
