@@ -1218,18 +1218,19 @@ struct QSieve2 {
 };
 
 template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>,
-          int GQ = NO_GLOBAL_QUEUES>
+          int GQ = NO_GLOBAL_QUEUES, bool DYN = false>
 __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride,
-                                               const RotConstsT<rot_row_bytes(R0)> *rc = nullptr) {
+                                               const RotConstsT<rot_row_bytes(R0)> *rc = nullptr,
+                                               unsigned int *slab = nullptr) {
   const double S59d = (double)a.kp.S59;
   const double MODd = (double)AP26_MOD;
   const double WRAP = MODd + TWO52;
-  /* as in qsieve_blocks: uniform loops, the whole wave in every step */
-  for (int64_t base = a.blk0; base < a.blk1; base += stride) {
-    const int64_t b = base + tid;
+  /* the 35 positions of block b (lanes with b >= blk1 work on a stand-in and
+     keep masks of zero); as in qsieve_blocks, uniform loops: the whole wave
+     in every step */
+  auto sieve_block = [&](int64_t b) {
     const bool has = b < a.blk1;
-    if (__ballot(has) == 0) break;
     double xd = (double)n53_of_block(a.kp, a.n43, has ? b : a.blk0) + TWO52;
     uint32_t init = has ? ~0u : 0u; /* lanes past the last block: masks of zero */
     /* opaque to the optimiser, which would otherwise turn the AND with init
@@ -1254,6 +1255,25 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       qs.template drain<0>();
       xd += S59d;
       xd = xd >= WRAP ? xd - MODd : xd;
+    }
+  };
+  if constexpr (DYN) {
+    /* `slab`: a workgroup counter in LDS, zero at the start of the item.
+       Each wave claims the next 64 blocks when it is ready for them instead
+       of taking every stride-th slab, so the waves of a workgroup reach the
+       end of the item together however their deeper levels fared. */
+    while (true) {
+      unsigned int s0 = 0;
+      if (qs.lane == 0) s0 = atomicAdd(slab, 64u);
+      s0 = __builtin_amdgcn_readfirstlane(s0); /* lane 0's claim, wave-uniform */
+      const int64_t base = a.blk0 + (int64_t)s0;
+      if (base >= a.blk1) break;
+      sieve_block(base + qs.lane);
+    }
+  } else {
+    for (int64_t base = a.blk0; base < a.blk1; base += stride) {
+      if (__ballot(base + tid < a.blk1) == 0) break;
+      sieve_block(base + tid);
     }
   }
   qs.template flush<0>();
@@ -1286,6 +1306,7 @@ sieve_persistent2_kernel(PersistArgs p) {
   constexpr int NPASS = AP26_NSHIFT / 2;
   __shared__ QSieve2Lds<NL, WAVES, LV, R0, Q0, GQ> L;
   __shared__ unsigned int s_item;
+  __shared__ unsigned int s_slab; /* blocks of the current item claimed by its waves */
   const RoundState rs = *p.rs;
   const int njobs = min(rs.per_round, rs.njobs - rs.base);
   const unsigned int nitems = (unsigned int)(njobs > 0 ? njobs : 0) * NPASS * p.nchunk;
@@ -1297,7 +1318,10 @@ sieve_persistent2_kernel(PersistArgs p) {
   if constexpr (R0) rc.init(threadIdx.x & 63);
   while (true) {
     __syncthreads();
-    if (threadIdx.x == 0) s_item = atomicAdd(p.work, 1u);
+    if (threadIdx.x == 0) {
+      s_item = atomicAdd(p.work, 1u);
+      s_slab = 0;
+    }
     __syncthreads();
     const unsigned int item = __builtin_amdgcn_readfirstlane(s_item); /* uniform: keeps job data scalar */
     if (item >= nitems) {
@@ -1347,7 +1371,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ>(qs, a, threadIdx.x, blockDim.x, &rc);
+    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ, true>(qs, a, threadIdx.x, blockDim.x, &rc, &s_slab);
   }
 }
 
