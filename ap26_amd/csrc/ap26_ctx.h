@@ -27,8 +27,9 @@ using namespace ap26;
 
 constexpr int64_t SURV_CAP = INT64_C(1) << 26; /* initial survivor buffer: 67M (~40M per K) */
 constexpr int SOL_CAP = 1 << 16;
-constexpr int NCOUNT = 8; /* device counters */
-enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_LIVE = 3, C_BIG = 4, C_ERR = 7 };
+constexpr int NCOUNT = 16; /* device counters */
+/* C_SLAB + p: blocks claimed by the sieve launch of shift pair p */
+enum { C_NSURV = 0, C_NTESTS = 1, C_NSOL = 2, C_LIVE = 3, C_BIG = 4, C_ERR = 7, C_SLAB = 8 };
 
 using SieveFn = void (*)(SieveArgs);
 

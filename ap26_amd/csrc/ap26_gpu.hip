@@ -274,6 +274,9 @@ static int enqueue_sieve(ap26_ctx *ctx, const SpanBufs &bf, const KParams &kp, i
     a.err = (unsigned int *)(bf.cnt + C_ERR);
     for (int v = 0; v < NVW; v++) a.vc[v] = vw_offset(a.shift0, VW_P[v]);
     a.gq = bf.gq; /* sized for `grid` workgroups of sv.block threads (ap26_open) */
+    /* the shift-pair sieves claim their blocks through a counter per launch,
+       cleared with the other counters (at most 5 passes of 2 shifts) */
+    a.slab = S == 2 && p < NCOUNT - C_SLAB ? (unsigned int *)(bf.cnt + C_SLAB + p) : nullptr;
     hipLaunchKernelGGL(sv.fn, dim3(grid), dim3(sv.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;

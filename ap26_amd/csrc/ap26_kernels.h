@@ -118,6 +118,7 @@ struct SieveArgs {
   unsigned int *__restrict__ err;   /* set if a queue step ran on part of a wave */
   int32_t vc[NVW];                  /* (shift0 mod p) + p*((p+1)/2), p = 61, 67, 71 */
   void *__restrict__ gq;            /* QEntry regions, one per wave of the grid (variants with GQ) */
+  unsigned int *__restrict__ slab;  /* plain path: blocks of [blk0, blk1) claimed so far (zero at launch) */
 };
 
 __host__ __device__ constexpr int32_t vw_offset(int32_t shift0, int p) { return shift0 % p + p * ((p + 1) / 2); }
@@ -1218,7 +1219,7 @@ struct QSieve2 {
 };
 
 template <int NL, class LV, int V = 0, int G = 0, int R0 = 0, int Q0 = 128, class TU = Tune<>,
-          int GQ = NO_GLOBAL_QUEUES, bool DYN = false>
+          int GQ = NO_GLOBAL_QUEUES, int DYN = 0>
 __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &qs, const SieveArgs &a, int64_t tid,
                                                int64_t stride,
                                                const RotConstsT<rot_row_bytes(R0)> *rc = nullptr,
@@ -1257,18 +1258,23 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       xd = xd >= WRAP ? xd - MODd : xd;
     }
   };
-  if constexpr (DYN) {
-    /* `slab`: a workgroup counter in LDS, zero at the start of the item.
-       Each wave claims the next 64 blocks when it is ready for them instead
-       of taking every stride-th slab, so the waves of a workgroup reach the
-       end of the item together however their deeper levels fared. */
+  if constexpr (DYN > 0) {
+    /* `slab`: a counter of claimed blocks, zero at the start -- in LDS for
+       the workgroup's item (DYN = 1) or in global memory for a launch's
+       whole grid (DYN = 4, fewer atomics).  Each wave claims the next DYN
+       slabs of 64 blocks when it is ready for them instead of taking every
+       stride-th slab, so the waves reach the end of the item (or launch)
+       together however their deeper levels fared. */
     while (true) {
       unsigned int s0 = 0;
-      if (qs.lane == 0) s0 = atomicAdd(slab, 64u);
+      if (qs.lane == 0) s0 = atomicAdd(slab, 64u * DYN);
       s0 = __builtin_amdgcn_readfirstlane(s0); /* lane 0's claim, wave-uniform */
       const int64_t base = a.blk0 + (int64_t)s0;
       if (base >= a.blk1) break;
-      sieve_block(base + qs.lane);
+      for (int k = 0; k < DYN; k++) {
+        if (base + 64 * k >= a.blk1) break;
+        sieve_block(base + 64 * k + qs.lane);
+      }
     }
   } else {
     for (int64_t base = a.blk0; base < a.blk1; base += stride) {
@@ -1371,7 +1377,7 @@ sieve_persistent2_kernel(PersistArgs p) {
     qs.gtb = (const char *)tab;
     qs.a = &a;
     qs.cur = &sc;
-    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ, true>(qs, a, threadIdx.x, blockDim.x, &rc, &s_slab);
+    qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ, 1>(qs, a, threadIdx.x, blockDim.x, &rc, &s_slab);
   }
 }
 
@@ -1407,8 +1413,8 @@ sieve_queue2_kernel(SieveArgs a) {
   qs.gtb = (const char *)a.tab;
   qs.a = &a;
   qs.cur = &sc;
-  qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
-                                           (int64_t)gridDim.x * blockDim.x, &rc);
+  qsieve2_blocks<NL, LV, V, G, R0, Q0, TU, GQ, 4>(qs, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                                              (int64_t)gridDim.x * blockDim.x, &rc, a.slab);
   close_survivors2(sc, a.surv, a.cap, threadIdx.x & 63);
 }
 
