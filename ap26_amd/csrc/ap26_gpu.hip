@@ -276,7 +276,11 @@ static int enqueue_sieve(ap26_ctx *ctx, const SpanBufs &bf, const KParams &kp, i
     a.gq = bf.gq; /* sized for `grid` workgroups of sv.block threads (ap26_open) */
     /* the shift-pair sieves claim their blocks through a counter per launch,
        cleared with the other counters (at most 5 passes of 2 shifts) */
-    a.slab = S == 2 && p < NCOUNT - C_SLAB ? (unsigned int *)(bf.cnt + C_SLAB + p) : nullptr;
+    a.slab = nullptr;
+    if (S == 2) {
+      if (p >= NCOUNT - C_SLAB) return AP26_E_INTERNAL;
+      a.slab = (unsigned int *)(bf.cnt + C_SLAB + p);
+    }
     hipLaunchKernelGGL(sv.fn, dim3(grid), dim3(sv.block), 0, ctx->stream, a);
     HIPCHK(hipGetLastError());
     if (st) st->sieve_launches++;
