@@ -85,7 +85,9 @@ PersistVariant make_pvariant2rg() {
    primes 61..251 in LDS).  92..102 use the 512-byte rotation rows
    (ROT_WIDE: no per-lane column registers or adds, twice the LDS; 0 = 95,
    the default: queues 1 and 2 in global memory, primes 61..233 in LDS);
-   16 waves still spill the rotation constants in the hot loop. */
+   16 waves still spill the rotation constants in the hot loop.  103, 104:
+   95 with the last 4 or 2 level-0 primes read through L1 (slower: the
+   random vector-memory gathers cost more than their LDS bank conflicts). */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -180,6 +182,9 @@ PersistVariant pick_pvariant(int id) {
     case 100: return make_pvariant2rw<34, 12, Levels<0, 20, 34, 55, 83>, Tune<2, 3, 1, 1>, 1>();
     case 101: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 1>();
     case 102: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 103, 104: 95 with the last 4 or 2 level-0 primes read through L1 */
+    case 103: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 4>, 1>();
+    case 104: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 2>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     default: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 95 */
   }

@@ -26,9 +26,9 @@ using namespace ap26;
    simple S=2, 10..30 = one-shift queued-sieve tuning candidates (LDS
    primes, waves per CU, levels, VGPR constants), 31..33 = shift-pair queued
    sieves without the rotation group, 34..36 = shift-pair with the first
-   1..3 level-0 primes from register bit windows instead of LDS, 40..47 =
+   1..3 level-0 primes from register bit windows instead of LDS, 40..49 =
    rotation-group forms (45: every queue in LDS, the allocation-free entry
-   point's sieve). */
+   point's sieve; 48, 49: 47 with packed level-0 primes through L1). */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -70,6 +70,9 @@ SieveVariant pick_variant(int id) {
     case 46: return make_q2variantrg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
     /* 47: 512-byte rotation rows, queues 1.. in global memory, 34 primes in LDS */
     case 47: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 48, 49: 47 with the last 4 or 2 level-0 primes read through L1 */
+    case 48: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 4>, 1>();
+    case 49: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 2>, 1>();
     default: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 47 */
   }
 }

@@ -892,20 +892,33 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
    lane whose position is only a stand-in) starts the AND chains, so such a
    lane's masks come out zero at no extra cost.  RBASE: byte offset of the
    rotation rows in the table area (after the packed tables). */
-template <int E, int SPLIT, int W, int RBASE>
-__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const RotConstsT<W> &rc, uint32_t init,
-                                           uint64_t &m0, uint64_t &m1) {
+template <int E, int SPLIT, int W, int RBASE, int GL = 0>
+__device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const char *gtb, const RotConstsT<W> &rc,
+                                           uint32_t init, uint64_t &m0, uint64_t &m1) {
   static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
   static_assert(W == 256 || W == 512, "row width");
   constexpr int PART = ROT_N / SPLIT;
   static_assert(E >= ROT_N, "level 0 holds the rotation group");
+  static_assert(GL >= 0 && GL <= E - ROT_N, "vector-memory primes are packed level-0 primes");
   const double x = xd - TWO52;
+  const uint32_t xs16 = lo32(xd) * 16u;
+  /* the last GL primes from the pair's global table, issued first so that
+     their longer latency runs under the rotated gathers; the byte offset is
+     non-negative (a centred entry within its prime's p entries) */
+  uint4 g[GL > 0 ? GL : 1];
+  if constexpr (GL > 0) {
+    static_for<E - GL, E>([&](auto IC) {
+      constexpr int I = decltype(IC)::value;
+      const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
+      constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
+      g[I - (E - GL)] = *(const uint4 *)(gtb + (uint32_t)(C0 + bo));
+    });
+  }
   /* W * x mod 2^32 (exact: W * 2^24 = 0 mod 2^32 for the i24 multiply-add)
      plus the offset of the rows' residue 0 (the rows' base and ROT_HALF
      rows); wide rows: plus this lane's column */
   const uint32_t xsw = lo32(xd) * (uint32_t)W + (uint32_t)(RBASE + W * ROT_HALF) +
                        (W == 512 ? 16u * (threadIdx.x & 15) : 0u);
-  const uint32_t xs16 = lo32(xd) * 16u;
   uint32_t a0 = init, a1 = init, a2 = init, a3 = init;
   /* SPLIT parts of the group, so that at most 16/SPLIT gathers are in flight */
   static_for<0, SPLIT>([&](auto HC) {
@@ -930,20 +943,29 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const Ro
       a3 &= w[k].w;
     });
   });
-  if constexpr (E > ROT_N) {
-    uint4 w[E - ROT_N];
-    static_for<ROT_N, E>([&](auto IC) {
+  if constexpr (E - GL > ROT_N) {
+    uint4 w[E - GL - ROT_N];
+    static_for<ROT_N, E - GL>([&](auto IC) {
       constexpr int I = decltype(IC)::value;
       const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
       constexpr int C0 = 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
       w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
     });
-    static_for<0, E - ROT_N>([&](auto IC) {
+    static_for<0, E - GL - ROT_N>([&](auto IC) {
       constexpr int k = decltype(IC)::value;
       a0 &= w[k].x;
       a1 &= w[k].y;
       a2 &= w[k].z;
       a3 &= w[k].w;
+    });
+  }
+  if constexpr (GL > 0) {
+    static_for<0, GL>([&](auto IC) {
+      constexpr int k = decltype(IC)::value;
+      a0 &= g[k].x;
+      a1 &= g[k].y;
+      a2 &= g[k].z;
+      a3 &= g[k].w;
     });
   }
   m0 = ((uint64_t)a1 << 32) | a0;
@@ -1102,6 +1124,17 @@ constexpr int PRIO_LEVEL0 = 2, PRIO_LEVEL1 = 3, PRIO_DEEP = 1;
 template <int P0 = PRIO_LEVEL0, int P1 = PRIO_LEVEL1, int PD = PRIO_DEEP, int SPLIT = 2>
 struct Tune {
   static constexpr int prio0 = P0, prio1 = P1, prio_deep = PD, split = SPLIT;
+  static constexpr int l1 = 0; /* level-0 primes read through vector memory */
+};
+
+/* As Tune, and the last L1 primes of level 0 (the packed ones after the
+   rotated group) are read from the pair's global table through the vector
+   memory path (L1) instead of LDS: their random 16-byte gathers are the
+   level-0 reads that meet bank conflicts, and the L1 is otherwise lightly
+   used. */
+template <int P0, int P1, int PD, int SPLIT, int L1>
+struct TuneL1 : Tune<P0, P1, PD, SPLIT> {
+  static constexpr int l1 = L1;
 };
 
 /* Queue capacity per wave: queue 0 holds Q0 entries, the deeper ones 128.
@@ -1242,8 +1275,8 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(TU::prio0);
       if constexpr (R0) {
-        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0), packed_lds_bytes<R0, NL>()>(xd, qs.ldsb, *rc, init, m0,
-                                                                                         m1);
+        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0), packed_lds_bytes<R0, NL>(), TU::l1>(
+            xd, qs.ldsb, qs.gtb, *rc, init, m0, m1);
       } else {
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
         m0 = has ? m0 : 0;
