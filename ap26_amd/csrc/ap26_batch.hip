@@ -185,6 +185,14 @@ PersistVariant pick_pvariant(int id) {
     /* 103, 104: 95 with the last 4 or 2 level-0 primes read through L1 */
     case 103: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 4>, 1>();
     case 104: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 2>, 1>();
+    /* 105: 95 with the packed level-0 gathers on live lanes only */
+    case 105: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
+    /* 106, 107: the same at 16 waves per CU (the registers now fit without
+       spills in the position loop), fewer primes in LDS for the queues */
+    case 106: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
+    case 107: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 2>, 1>();
+    case 108: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 4>, 1>();
+    case 109: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<1, 2, 0, 2>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     default: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 95 */
   }

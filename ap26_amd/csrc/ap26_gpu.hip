@@ -73,6 +73,11 @@ SieveVariant pick_variant(int id) {
     /* 48, 49: 47 with the last 4 or 2 level-0 primes read through L1 */
     case 48: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 4>, 1>();
     case 49: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 2>, 1>();
+    /* 50: 47 with the packed level-0 gathers on live lanes only */
+    case 50: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
+    /* 51, 52: the same at 16 waves per CU */
+    case 51: return make_q2variantrw<30, 16, Levels<0, 20, 30, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
+    case 52: return make_q2variantrw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 2>, 1>();
     default: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 47 */
   }
 }

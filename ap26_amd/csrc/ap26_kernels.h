@@ -892,7 +892,7 @@ __device__ __forceinline__ uint64_t apply_filters2(uint64_t m, double xd, const 
    lane whose position is only a stand-in) starts the AND chains, so such a
    lane's masks come out zero at no extra cost.  RBASE: byte offset of the
    rotation rows in the table area (after the packed tables). */
-template <int E, int SPLIT, int W, int RBASE, int GL = 0>
+template <int E, int SPLIT, int W, int RBASE, int GL = 0, bool SKIP = false>
 __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const char *gtb, const RotConstsT<W> &rc,
                                            uint32_t init, uint64_t &m0, uint64_t &m1) {
   static_assert(ROT_N % SPLIT == 0, "equal parts of the rotation group");
@@ -944,20 +944,27 @@ __device__ __forceinline__ void level0_rot(double xd, const char *ldsb, const ch
     });
   });
   if constexpr (E - GL > ROT_N) {
-    uint4 w[E - GL - ROT_N];
-    static_for<ROT_N, E - GL>([&](auto IC) {
-      constexpr int I = decltype(IC)::value;
-      const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
-      constexpr int C0 = 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
-      w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
-    });
-    static_for<0, E - GL - ROT_N>([&](auto IC) {
-      constexpr int k = decltype(IC)::value;
-      a0 &= w[k].x;
-      a1 &= w[k].y;
-      a2 &= w[k].z;
-      a3 &= w[k].w;
-    });
+    auto packed = [&]() {
+      uint4 w[E - GL - ROT_N];
+      static_for<ROT_N, E - GL>([&](auto IC) {
+        constexpr int I = decltype(IC)::value;
+        const int bo = centred_byteoff<PT.p[I], 16>(x, xs16);
+        constexpr int C0 = 16 * (PT.off[I] - PT.off[ROT_N] + (PT.p[I] - 1) / 2);
+        w[I - ROT_N] = *(const uint4 *)(ldsb + C0 + bo);
+      });
+      static_for<0, E - GL - ROT_N>([&](auto IC) {
+        constexpr int k = decltype(IC)::value;
+        a0 &= w[k].x;
+        a1 &= w[k].y;
+        a2 &= w[k].z;
+        a3 &= w[k].w;
+      });
+    };
+    if constexpr (SKIP) {
+      if ((a0 | a1 | a2 | a3) != 0u) packed(); /* a zero mask stays zero */
+    } else {
+      packed();
+    }
   }
   if constexpr (GL > 0) {
     static_for<0, GL>([&](auto IC) {
@@ -1124,7 +1131,8 @@ constexpr int PRIO_LEVEL0 = 2, PRIO_LEVEL1 = 3, PRIO_DEEP = 1;
 template <int P0 = PRIO_LEVEL0, int P1 = PRIO_LEVEL1, int PD = PRIO_DEEP, int SPLIT = 2>
 struct Tune {
   static constexpr int prio0 = P0, prio1 = P1, prio_deep = PD, split = SPLIT;
-  static constexpr int l1 = 0; /* level-0 primes read through vector memory */
+  static constexpr int l1 = 0;          /* level-0 primes read through vector memory */
+  static constexpr bool lane_skip = false; /* packed level-0 gathers only on live lanes */
 };
 
 /* As Tune, and the last L1 primes of level 0 (the packed ones after the
@@ -1135,6 +1143,15 @@ struct Tune {
 template <int P0, int P1, int PD, int SPLIT, int L1>
 struct TuneL1 : Tune<P0, P1, PD, SPLIT> {
   static constexpr int l1 = L1;
+};
+
+/* As Tune, and the packed level-0 gathers (after the rotated group) run only
+   on the lanes whose pair mask is still non-zero: the LDS resolves bank
+   conflicts among the active lanes only, at the price of waiting for the
+   rotated group's result before those gathers are issued. */
+template <int P0, int P1, int PD, int SPLIT>
+struct TuneSkip : Tune<P0, P1, PD, SPLIT> {
+  static constexpr bool lane_skip = true;
 };
 
 /* Queue capacity per wave: queue 0 holds Q0 entries, the deeper ones 128.
@@ -1275,7 +1292,7 @@ __device__ __forceinline__ void qsieve2_blocks(QSieve2<NL, LV, R0, Q0, TU, GQ> &
       uint64_t m0, m1;
       __builtin_amdgcn_s_setprio(TU::prio0);
       if constexpr (R0) {
-        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0), packed_lds_bytes<R0, NL>(), TU::l1>(
+        level0_rot<LV::B[1], TU::split, rot_row_bytes(R0), packed_lds_bytes<R0, NL>(), TU::l1, TU::lane_skip>(
             xd, qs.ldsb, qs.gtb, *rc, init, m0, m1);
       } else {
         level0_pair<0, LV::B[1], V, G>(xd, qs.ldsb, qs.gtb, m0, m1, a.kp, a.vc);
