@@ -54,14 +54,16 @@ SPANS = [(366384, 0, 0, 10, 0, 3), (875859, 640, 0, 10, 100, 102), (48620635, 64
          (600000000, 16912, 0, 10, 4000, 4002)]  # largest K and SHIFT: terms just below 2^63
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 23, 31, 40, 42, 45, 46, 48])
+@pytest.mark.parametrize("variant", [0, 1, 2, 23, 31, 40, 42, 45, 46, 48, 50, 52])
 @pytest.mark.parametrize("span", SPANS)
 def test_sieve_span_parity(variant, span):
     """Plain-path sieve variants (0 = 47, the rotated default with 512-byte
     rotation rows, queues 1 and 2 in global memory and 34 primes in LDS;
     46 = 256-byte rows, the last queue in global memory, 37 primes in LDS;
     45 = every queue in LDS, the allocation-free entry point's; 48 = 47 with
-    the last 4 level-0 primes read through L1 instead of LDS; 31 = the
+    the last 4 level-0 primes read through L1 instead of LDS; 50 = 47 with
+    the packed level-0 gathers on live lanes only, 52 = the same at 16 waves
+    per CU; 31 = the
     shift-pair sieve without the rotation group, 40 = rotation group with
     level 0 on exactly its 16 primes, 42 = on 18 primes, 1/2 = the simple
     per-lane sieves, 23 = one-shift queued sieve) keep exactly the oracle's
