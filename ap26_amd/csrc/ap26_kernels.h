@@ -189,7 +189,7 @@ static __global__ void __launch_bounds__(256) sieve_kernel(SieveArgs a) {
             for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(ldsb + C0 + bo + 8 * s);
           } else {
 #pragma unroll
-            for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(gtb + (int64_t)(C0 + bo + 8 * s));
+            for (int s = 0; s < S; s++) m[s] &= *(const uint64_t *)(gtb + (uint32_t)(C0 + bo + 8 * s));
           }
           if constexpr (check_after(I) || I == NSP - 1) {
             uint64_t any = 0;
@@ -297,8 +297,8 @@ __device__ __forceinline__ uint64_t apply_primes(uint64_t m, double xd, const ch
     constexpr int C0 = 8 * (PT.off[I] + (P - 1) / 2);
     if constexpr (I < NL && !(I < 32 && ((G >> I) & 1)))
       w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
-    else
-      w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
+    else /* C0 + bo >= 0: an unsigned offset from the wave-uniform base */
+      w[I - B] = *(const uint64_t *)(gtb + (uint32_t)(C0 + bo[I - B]));
   });
   static_for<B, E>([&](auto IC) { m &= w[decltype(IC)::value - B]; });
   return m;
@@ -861,8 +861,12 @@ __device__ __forceinline__ uint64_t apply_primes2(uint64_t m, double xd, const c
         constexpr int C0 = 16 * (PT.off[I] - PT.off[rot_primes(R0)] + (PT.p[I] - 1) / 2);
         w[I - B] = *(const uint64_t *)(ldsb + C0 + bo[I - B]);
       } else {
+        /* C0 + bo is a byte offset within the prime's p entries, never
+           negative: as an unsigned 32-bit offset from the wave-uniform table
+           base it takes one add (the scalar-base form of the load) instead
+           of a 64-bit sign extension and add */
         constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
-        w[I - B] = *(const uint64_t *)(gtb + (int64_t)(C0 + bo[I - B]));
+        w[I - B] = *(const uint64_t *)(gtb + (uint32_t)(C0 + bo[I - B]));
       }
     });
     if constexpr (E <= NL) interleave_residues_and_reads<E - B>();
@@ -1038,7 +1042,7 @@ __device__ __forceinline__ void level0_pair(double xd, const char *ldsb, const c
     constexpr int I = decltype(IC)::value;
     constexpr int C0 = 16 * (PT.off[I] + (PT.p[I] - 1) / 2);
     if constexpr (I >= E - G)
-      w[I - B] = *(const uint4 *)(gtb + (int64_t)(C0 + bo[I - B]));
+      w[I - B] = *(const uint4 *)(gtb + (uint32_t)(C0 + bo[I - B]));
     else
       w[I - B] = *(const uint4 *)(ldsb + C0 + bo[I - B]);
   });
