@@ -1217,10 +1217,21 @@ struct QSieve2 {
       return qb + (QI == 0 ? 0 : Q0 + 128 * (QI - 1));
   }
 
+  /* Entry idx (0 <= idx < the queue's capacity) of queue QI; in global
+     memory as an unsigned 32-bit byte offset from the wave-uniform base, so
+     that the access takes the scalar-base form (no 64-bit address add). */
+  template <int QI>
+  __device__ __forceinline__ QEntry &qe(int idx) const {
+    if constexpr (QI >= GQ)
+      return *(QEntry *)((char *)gqb + (uint32_t)((128 * (QI - GQ) + idx) * (int)sizeof(QEntry)));
+    else
+      return q<QI>()[idx];
+  }
+
   template <int QI>
   __device__ __forceinline__ void push(double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
-    if (pred) q<QI>()[qn[QI] + lanes_below(bal)] = QEntry{xd, m};
+    if (pred) qe<QI>(qn[QI] + lanes_below(bal)) = QEntry{xd, m};
     qn[QI] += __popcll(bal);
     __builtin_amdgcn_wave_barrier();
   }
@@ -1243,7 +1254,7 @@ struct QSieve2 {
     while (qn[L] >= 64) {
       check_full_wave();
       qn[L] -= 64;
-      const QEntry e = q<L>()[qn[L] + lane];
+      const QEntry e = qe<L>(qn[L] + lane);
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_s_setprio(L == 0 ? TU::prio1 : TU::prio_deep);
       run_level<L>(e, true);
@@ -1260,7 +1271,7 @@ struct QSieve2 {
         check_full_wave();
         const bool pred = lane < n;
         QEntry e{TWO52, 0};
-        if (pred) e = q<L>()[lane];
+        if (pred) e = qe<L>(lane);
         __builtin_amdgcn_wave_barrier();
         qn[L] = 0;
         run_level<L>(e, pred);
