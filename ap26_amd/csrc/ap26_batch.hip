@@ -83,11 +83,16 @@ PersistVariant make_pvariant2rg() {
    LDS on more table primes: a level that reads its tables through L1/L2
    costs more than the queue traffic (85: the last queue in global memory,
    primes 61..251 in LDS).  92..102 use the 512-byte rotation rows
-   (ROT_WIDE: no per-lane column registers or adds, twice the LDS; 0 = 95,
-   the default: queues 1 and 2 in global memory, primes 61..233 in LDS);
-   16 waves still spill the rotation constants in the hot loop.  103, 104:
-   95 with the last 4 or 2 level-0 primes read through L1 (slower: the
-   random vector-memory gathers cost more than their LDS bank conflicts). */
+   (ROT_WIDE: no per-lane column registers or adds, twice the LDS; 95:
+   queues 1 and 2 in global memory, primes 61..233 in LDS); 16 waves still
+   spill the rotation constants in the hot loop.  103, 104: 95 with the
+   last 4 or 2 level-0 primes read through L1 (slower: the random
+   vector-memory gathers cost more than their LDS bank conflicts).
+   110..127: 95 with more compaction levels (queues 1.. in global memory):
+   an entry whose mask has died leaves before the next run of lookups, so
+   each extra level saves lookups for the price of one queue step; 0 = 117,
+   the default: seven levels, 61..157 | 163..197 | 199..233 | 239..277 |
+   281..337 | 347..431 | 433..541. */
 PersistVariant pick_pvariant(int id) {
   switch (id) {
     case 1: return make_pvariant<31, 16, Levels4>();
@@ -193,8 +198,31 @@ PersistVariant pick_pvariant(int id) {
     case 107: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 2>, 1>();
     case 108: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 4>, 1>();
     case 109: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<1, 2, 0, 2>, 1>();
+    /* 110..113: 95 with five levels (one more compaction, its queue in
+       global memory) */
+    case 110: return make_pvariant2rw<34, 12, Levels<0, 20, 26, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 111: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 112: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 42, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 113: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 114..118: six and seven levels */
+    case 114: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 115: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 116: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 117: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 118: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 34, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 119..123: eight to ten levels */
+    case 119: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 60, 70, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 120: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 40, 46, 51, 60, 70, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 121: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 58, 66, 74, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 122: return make_pvariant2rw<34, 12, Levels<0, 20, 27, 31, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 123: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 40, 46, 53, 61, 71, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 124..127: split points near 117's */
+    case 124: return make_pvariant2rw<34, 12, Levels<0, 20, 29, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 125: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 43, 53, 68, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 126: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 41, 49, 63, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 127: return make_pvariant2rw<34, 12, Levels<0, 21, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
-    default: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 95 */
+    default: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 117 */
   }
 }
 
@@ -1118,7 +1146,7 @@ extern "C" int ap26_server_stats(ap26_server *s, int64_t *units_done, int64_t *k
    a path is given, writes it to a cache file that ap26_open loads again when
    AP26_TUNE_CACHE names it. */
 
-static const int TUNE_VARIANTS[] = {95, 85, 98, 101, 100, 86, 90, 88, 70, 75, 54, 28};
+static const int TUNE_VARIANTS[] = {117, 126, 119, 95, 121, 98, 101, 85, 100, 70, 75, 28};
 static const int TUNE_CHUNKS[] = {11, 21, 43, 85};
 
 extern "C" int ap26_autotune(ap26_ctx *ctx, int32_t K, int32_t SHIFT, int32_t quick, const char *cache_path,

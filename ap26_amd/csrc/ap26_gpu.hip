@@ -20,15 +20,17 @@ using namespace ap26;
 
 #include "ap26_ctx.h"
 
-/* Sieve variants: 0 = default (47: shift-pair queued sieve with the
-   bank-rotated level-0 group in 512-byte rows, level 0 on 20 primes, 12
-   waves per CU, queues 1 and 2 in global memory), 1 = simple S=1, 2 =
+/* Sieve variants: 0 = default (53: shift-pair queued sieve with the
+   bank-rotated level-0 group in 512-byte rows, level 0 on 20 primes, seven
+   compaction levels, 12 waves per CU, queues 1.. in global memory), 1 =
+   simple S=1, 2 =
    simple S=2, 10..30 = one-shift queued-sieve tuning candidates (LDS
    primes, waves per CU, levels, VGPR constants), 31..33 = shift-pair queued
    sieves without the rotation group, 34..36 = shift-pair with the first
-   1..3 level-0 primes from register bit windows instead of LDS, 40..49 =
+   1..3 level-0 primes from register bit windows instead of LDS, 40..53 =
    rotation-group forms (45: every queue in LDS, the allocation-free entry
-   point's sieve; 48, 49: 47 with packed level-0 primes through L1). */
+   point's sieve; 47: four levels; 48, 49: 47 with packed level-0 primes
+   through L1; 50..52: live-lane packed gathers, at 12 and 16 waves). */
 SieveVariant pick_variant(int id) {
   switch (id) {
     case 1: return make_variant<1, 45>();
@@ -78,7 +80,9 @@ SieveVariant pick_variant(int id) {
     /* 51, 52: the same at 16 waves per CU */
     case 51: return make_q2variantrw<30, 16, Levels<0, 20, 30, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
     case 52: return make_q2variantrw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 2>, 1>();
-    default: return make_q2variantrw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 47 */
+    /* 53: 47 with seven compaction levels (the batched engine's 117) */
+    case 53: return make_q2variantrw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    default: return make_q2variantrw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 53 */
   }
 }
 

@@ -110,7 +110,7 @@ VALU_ISSUE_CEILING = 1.256
 TRAFFIC_BYTES_PER_K = 1.0464e9
 # The batched engine's sieve when no variant is named (pick_pvariant(0) in
 # ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
-DEFAULT_BATCH_VARIANT = 95
+DEFAULT_BATCH_VARIANT = 117
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
 # n43 starts (10,840 x 8 B) read once.

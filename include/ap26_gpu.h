@@ -52,11 +52,11 @@ typedef struct {
 
 /* Tuning knobs (0 = library default / autotuner cache). */
 typedef struct {
-  int32_t sieve_variant;    /* 0 default (47: shift-pair queued-compaction
+  int32_t sieve_variant;    /* 0 default (53: shift-pair queued-compaction
                                sieve with the bank-rotated level-0 group in
-                               512-byte rows), 1 simple per-shift sieve, 2
-                               simple sieve with 2 shifts per pass, 10..52
-                               tuning candidates */
+                               512-byte rows, seven levels), 1 simple
+                               per-shift sieve, 2 simple sieve with 2 shifts
+                               per pass, 10..53 tuning candidates */
   int32_t block;            /* (reported) threads per workgroup of the sieve */
   int32_t blocks_per_cu;    /* resident workgroups per CU (grid = CUs x this) */
   int32_t verbose;

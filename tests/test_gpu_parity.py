@@ -54,11 +54,12 @@ SPANS = [(366384, 0, 0, 10, 0, 3), (875859, 640, 0, 10, 100, 102), (48620635, 64
          (600000000, 16912, 0, 10, 4000, 4002)]  # largest K and SHIFT: terms just below 2^63
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 23, 31, 40, 42, 45, 46, 48, 50, 52])
+@pytest.mark.parametrize("variant", [0, 1, 2, 23, 31, 40, 42, 45, 46, 47, 48, 50, 52])
 @pytest.mark.parametrize("span", SPANS)
 def test_sieve_span_parity(variant, span):
-    """Plain-path sieve variants (0 = 47, the rotated default with 512-byte
-    rotation rows, queues 1 and 2 in global memory and 34 primes in LDS;
+    """Plain-path sieve variants (0 = 53, the rotated default with 512-byte
+    rotation rows, seven compaction levels, queues 1.. in global memory and
+    34 primes in LDS; 47 = the same with four levels;
     46 = 256-byte rows, the last queue in global memory, 37 primes in LDS;
     45 = every queue in LDS, the allocation-free entry point's; 48 = 47 with
     the last 4 level-0 primes read through L1 instead of LDS; 50 = 47 with
