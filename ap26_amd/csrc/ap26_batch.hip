@@ -221,6 +221,15 @@ PersistVariant pick_pvariant(int id) {
     case 125: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 43, 53, 68, 83>, Tune<2, 3, 1, 1>, 1>();
     case 126: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 41, 49, 63, 83>, Tune<2, 3, 1, 1>, 1>();
     case 127: return make_pvariant2rw<34, 12, Levels<0, 21, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    /* 128..131: 117 with a level 0 of 19 primes, or other wave priorities */
+    case 128: return make_pvariant2rw<34, 12, Levels<0, 19, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 129: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 2, 1>, 1>();
+    case 130: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 0, 1>, 1>();
+    case 131: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<1, 3, 1, 1>, 1>();
+    /* 132..134: 117 with the global queues from 3, 2 or 4 on non-temporal */
+    case 132: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 3>, 1>();
+    case 133: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 2>, 1>();
+    case 134: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 4>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     default: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 117 */
   }

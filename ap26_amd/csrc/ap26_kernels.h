@@ -1137,6 +1137,7 @@ struct Tune {
   static constexpr int prio0 = P0, prio1 = P1, prio_deep = PD, split = SPLIT;
   static constexpr int l1 = 0;          /* level-0 primes read through vector memory */
   static constexpr bool lane_skip = false; /* packed level-0 gathers only on live lanes */
+  static constexpr int nt_queue = 99;    /* global queues from this one on: non-temporal */
 };
 
 /* As Tune, and the last L1 primes of level 0 (the packed ones after the
@@ -1147,6 +1148,14 @@ struct Tune {
 template <int P0, int P1, int PD, int SPLIT, int L1>
 struct TuneL1 : Tune<P0, P1, PD, SPLIT> {
   static constexpr int l1 = L1;
+};
+
+/* As Tune, and the global queues NTQ.. (the deep ones, which fill slowly)
+   are written and read with non-temporal hints, so that their lines are the
+   first the L2 evicts and the busier queues' lines stay. */
+template <int P0, int P1, int PD, int SPLIT, int NTQ>
+struct TuneNT : Tune<P0, P1, PD, SPLIT> {
+  static constexpr int nt_queue = NTQ;
 };
 
 /* As Tune, and the packed level-0 gathers (after the rotated group) run only
@@ -1229,9 +1238,30 @@ struct QSieve2 {
   }
 
   template <int QI>
+  __device__ __forceinline__ void put(int idx, double xd, uint64_t m) const {
+    if constexpr (QI >= GQ && QI >= TU::nt_queue) {
+      QEntry &e = qe<QI>(idx);
+      __builtin_nontemporal_store(xd, &e.xd);
+      __builtin_nontemporal_store(m, &e.m);
+    } else {
+      qe<QI>(idx) = QEntry{xd, m};
+    }
+  }
+
+  template <int QI>
+  __device__ __forceinline__ QEntry get(int idx) const {
+    if constexpr (QI >= GQ && QI >= TU::nt_queue) {
+      const QEntry &e = qe<QI>(idx);
+      return QEntry{__builtin_nontemporal_load(&e.xd), __builtin_nontemporal_load(&e.m)};
+    } else {
+      return qe<QI>(idx);
+    }
+  }
+
+  template <int QI>
   __device__ __forceinline__ void push(double xd, uint64_t m, bool pred) {
     const uint64_t bal = __ballot(pred);
-    if (pred) qe<QI>(qn[QI] + lanes_below(bal)) = QEntry{xd, m};
+    if (pred) put<QI>(qn[QI] + lanes_below(bal), xd, m);
     qn[QI] += __popcll(bal);
     __builtin_amdgcn_wave_barrier();
   }
@@ -1254,7 +1284,7 @@ struct QSieve2 {
     while (qn[L] >= 64) {
       check_full_wave();
       qn[L] -= 64;
-      const QEntry e = qe<L>(qn[L] + lane);
+      const QEntry e = get<L>(qn[L] + lane);
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_s_setprio(L == 0 ? TU::prio1 : TU::prio_deep);
       run_level<L>(e, true);
@@ -1271,7 +1301,7 @@ struct QSieve2 {
         check_full_wave();
         const bool pred = lane < n;
         QEntry e{TWO52, 0};
-        if (pred) e = qe<L>(lane);
+        if (pred) e = get<L>(lane);
         __builtin_amdgcn_wave_barrier();
         qn[L] = 0;
         run_level<L>(e, pred);

@@ -97,17 +97,17 @@ PEAK_HBM_GBS = 8000.0
 # peak above assumes 1.0.  Rates and ceiling are both wall-time rates at the
 # nominal 2.4 GHz (the counters show the clock at 2.20-2.36 GHz under the
 # sieve's load, box to box, so per-clock rates are 2-9 % higher).
-SIEVE_VALU_LANE_INSTR_PER_K = 3.5003e12
+SIEVE_VALU_LANE_INSTR_PER_K = 3.3361e12
 PRP_VALU_LANE_INSTR_PER_SQUARING = 41.37
 VALU_ISSUE_CEILING = 1.256
 # HBM bytes per searched K moved by the persistent sieve kernel, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) over a 16-K round of this
 # workload, each scaled by what the same counter reads for prp_kernel over
 # 2^26 values, which reads 512 MiB and writes 64 MiB exactly
-# (profiles/sieve_rot_traffic.md): (2 x 175,290 + 15,999,148) KiB / 16 K,
-# the excess over the algorithmic bytes the global-memory queues written back
-# from L2 (their share varies from box to box).
-TRAFFIC_BYTES_PER_K = 1.0464e9
+# (profiles/sieve_rot_traffic.md): (2 x 4,169,556 + 536,497,536) KiB / 16 K,
+# the excess over the algorithmic bytes the per-wave global-memory queues of
+# the deeper compaction levels, written back from L2 and read again.
+TRAFFIC_BYTES_PER_K = 3.4869e10
 # The batched engine's sieve when no variant is named (pick_pvariant(0) in
 # ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
 DEFAULT_BATCH_VARIANT = 117
