@@ -230,6 +230,11 @@ PersistVariant pick_pvariant(int id) {
     case 132: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 3>, 1>();
     case 133: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 2>, 1>();
     case 134: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 4>, 1>();
+    /* 135..137: level 0 on the 16 rotated primes only, the packed 139..157
+       a level of their own (slower: twice the queue-0 traffic) */
+    case 135: return make_pvariant2rw<34, 12, Levels<0, 16, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
+    case 136: return make_pvariant2rw<34, 12, Levels<0, 16, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 3, 1>, 1>();
+    case 137: return make_pvariant2rw<34, 12, Levels<0, 16, 22, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
     case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
     default: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 117 */
   }
