@@ -83,12 +83,13 @@ def test_batch_segments_equal_one_segment(gpu_ctx, golden_index, monkeypatch):
         assert d1[f] == d0[f], f
 
 
-@pytest.mark.parametrize("variant", ["0", "95", "126", "85", "75", "76", "98", "92", "103", "105", "107", "70", "54",
-                                     "52", "56", "67", "28", "10", "9", "12", "13", "32"])
+@pytest.mark.parametrize("variant", ["0", "95", "126", "132", "138", "85", "75", "76", "98", "92", "103", "105", "107",
+                                     "70", "54", "52", "56", "67", "28", "10", "9", "12", "13", "32"])
 def test_batch_sieve_matches_plain_survivors(variant, monkeypatch):
     """The batched sieve (shift-pair kernels with and without the bank-rotated
     level-0 group -- 0 = 117 (seven compaction levels), 126 (seven, other
-    splits), 95 (four), 98, 92 with 512-byte rotation rows and deeper
+    splits), 132 (117 with non-temporal deep queues), 138 (seven levels with
+    the level-1 queue in LDS), 95 (four), 98, 92 with 512-byte rotation rows and deeper
     queues in global memory, 103 = 95 with the last 4 level-0 primes read
     through L1, 105 = 95 with the packed level-0 gathers on live lanes
     only, 107 = the same at 16 waves per CU, 85, 75, 76 with deeper queues in global memory
