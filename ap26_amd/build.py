@@ -49,6 +49,7 @@ def _atomic_link(tmp, dst):
 def build_gpu(verbose=False):
     """Compile the HIP sources to objects in parallel, then link the library."""
     srcs = [os.path.join(CSRC, "ap26_gpu.hip"), os.path.join(CSRC, "ap26_batch.hip")]
+    srcs += [os.path.join(CSRC, f"ap26_pvariants_{i}.hip") for i in range(1, 6)]
     srcs = [s for s in srcs if os.path.exists(s)]
     objs = [os.path.join(PKG, os.path.basename(s) + ".o") for s in srcs]
     cmds = [[HIPCC, *HIP_FLAGS, "-c", "-o", o, s] for s, o in zip(srcs, objs)]

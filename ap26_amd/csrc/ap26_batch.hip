@@ -18,6 +18,7 @@
 
 #include "ap26_ctx.h"
 #include "ap26_host_math.h"
+#include "ap26_pvariants.h"
 
 using namespace ap26;
 
@@ -31,218 +32,16 @@ constexpr int DEFAULT_CHUNK = 11;
 constexpr int BATCH_SEG_K = 2048; /* searched K per segment of one batched call */
 constexpr int MAX_RANGES = 1 << 20;
 
-using PersistFn = void (*)(PersistArgs);
-
-struct PersistVariant {
-  int NL, waves, S;
-  PersistFn fn;
-  int gq_entries = 0; /* global queue entries per wave (PersistArgs::gq) */
-};
-
-template <int NL, int WAVES, class LV>
-PersistVariant make_pvariant() {
-  return PersistVariant{NL, WAVES, 1, &sieve_persistent_kernel<NL, WAVES, LV>};
-}
-
-template <int NL, int WAVES, class LV, int V = 0, int G = 0>
-PersistVariant make_pvariant2() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, V, G>};
-}
-
-/* shift pairs with the bank-rotated level-0 group (primes 61..137) */
-template <int NL, int WAVES, class LV, class TU = Tune<>>
-PersistVariant make_pvariant2r() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU>};
-}
-
-/* the rotated form with 512-byte rows (no per-lane column registers) */
-template <int NL, int WAVES, class LV, class TU, int GQ>
-PersistVariant make_pvariant2rw() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_WIDE, TU, GQ>,
-                        qsieve2_global_entries<LV, GQ>()};
-}
-
-/* the rotated form with queues GQ.. in global memory, so that more primes fit in LDS */
-template <int NL, int WAVES, class LV, class TU, int GQ>
-PersistVariant make_pvariant2rg() {
-  return PersistVariant{NL, WAVES, 2, &sieve_persistent2_kernel<NL, WAVES, LV, 0, 0, ROT_N, TU, GQ>,
-                        qsieve2_global_entries<LV, GQ>()};
-}
-
-/* 1..3, 9: one shift per item; 10..40, 50..56: shift pairs.  Level 0
-   (every position, both shifts of a pair per 16-byte read) runs on the first
-   16..20 primes: a pair's 128-bit mask stays non-zero longer than one shift's,
-   so level 0 pays off further than on one shift (61..131).  35..38: the first
-   1..3 level-0 primes from register bit windows; 39..40: the last 2 or 4
-   level-0 primes read through L1 instead of LDS.  50..91: level 0 opens
-   with the bank-rotated group of 61..137 (level0_rot), whose 64 VGPRs of
-   per-lane constants leave room for 12 waves per CU (16 waves spill in the
-   hot loop, 8 lose the latency hiding); 57..74 vary the level boundaries,
-   the wave priorities and how many rotated gathers are in flight at once
-   (Tune).  75..91 move the deeper queues to global memory and spend the
-   LDS on more table primes: a level that reads its tables through L1/L2
-   costs more than the queue traffic (85: the last queue in global memory,
-   primes 61..251 in LDS).  92..102 use the 512-byte rotation rows
-   (ROT_WIDE: no per-lane column registers or adds, twice the LDS; 95:
-   queues 1 and 2 in global memory, primes 61..233 in LDS); 16 waves still
-   spill the rotation constants in the hot loop.  103, 104: 95 with the
-   last 4 or 2 level-0 primes read through L1 (slower: the random
-   vector-memory gathers cost more than their LDS bank conflicts).
-   110..127: 95 with more compaction levels (queues 1.. in global memory):
-   an entry whose mask has died leaves before the next run of lookups, so
-   each extra level saves lookups for the price of one queue step; 0 = 117,
-   the default: seven levels, 61..157 | 163..197 | 199..233 | 239..277 |
-   281..337 | 347..431 | 433..541. */
+/* The variant `id` of the persistent sieve (ap26_pvariants.h); 0 or an
+   unknown id: 117, the default. */
 PersistVariant pick_pvariant(int id) {
-  switch (id) {
-    case 1: return make_pvariant<31, 16, Levels4>();
-    case 2: return make_pvariant<45, 12, Levels4>();
-    case 3: return make_pvariant<25, 16, Levels5>();
-    case 10: return make_pvariant2<29, 16, Levels4>();
-    case 11: return make_pvariant2<31, 14, Levels4>();
-    case 12: return make_pvariant2<35, 12, Levels4>();
-    case 13: return make_pvariant2<20, 12, Levels5>();
-    case 14: return make_pvariant2<31, 12, Levels4>();
-    case 9: return make_pvariant<38, 16, Levels4>();
-    case 15: return make_pvariant2<20, 16, Levels4>();
-    case 16: return make_pvariant2<24, 16, Levels4>();
-    case 17: return make_pvariant2<18, 16, Levels4>();
-    case 18: return make_pvariant2<22, 16, Levels4>();
-    case 19: return make_pvariant2<15, 16, Levels4>();
-    case 20: return make_pvariant2<18, 16, Levels<0, 15, 22, 29, 45, 83>>();
-    case 21: return make_pvariant2<15, 16, Levels<0, 15, 22, 29, 45, 83>>();
-    case 22: return make_pvariant2<29, 16, Levels<0, 18, 29, 45, 83>>();
-    case 23: return make_pvariant2<29, 16, Levels<0, 20, 29, 45, 83>>();
-    case 24: return make_pvariant2<29, 16, Levels<0, 22, 29, 45, 83>>();
-    case 25: return make_pvariant2<29, 16, Levels<0, 20, 29, 41, 83>>();
-    case 26: return make_pvariant2<29, 16, Levels<0, 20, 29, 51, 83>>();
-    case 27: return make_pvariant2<29, 16, Levels<0, 24, 29, 45, 83>>();
-    case 29: return make_pvariant2<29, 16, Levels<0, 19, 29, 45, 83>>();
-    case 30: return make_pvariant2<29, 16, Levels<0, 20, 29, 55, 83>>();
-    case 31: return make_pvariant2<29, 16, Levels<0, 18, 27, 45, 83>>();
-    case 32: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>>();
-    case 33: return make_pvariant2<27, 16, Levels<0, 20, 29, 51, 83>>();
-    case 34: return make_pvariant2<29, 16, Levels<0, 19, 29, 51, 83>>();
-    case 35: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 1>();
-    case 36: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 2>();
-    case 37: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 3>();
-    case 38: return make_pvariant2<29, 16, Levels<0, 20, 31, 51, 83>, 3>();
-    case 39: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 2>();
-    case 40: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>, 0, 4>();
-    case 50: return make_pvariant2r<21, 16, Levels<0, 16, 21, 51, 83>>();
-    case 51: return make_pvariant2r<21, 16, Levels<0, 18, 21, 51, 83>>();
-    case 52: return make_pvariant2r<29, 12, Levels<0, 16, 29, 51, 83>>();
-    case 53: return make_pvariant2r<29, 12, Levels<0, 18, 29, 51, 83>>();
-    case 54: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>>();
-    case 55: return make_pvariant2r<29, 12, Levels<0, 20, 29, 51, 83>>();
-    case 56: return make_pvariant2r<36, 8, Levels<0, 18, 36, 55, 83>>();
-    /* 57..66: variant 54 with other wave priorities (level 0, level-1
-       drains, deeper drains) and level-0 splits */
-    case 57: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<0, 0, 0, 2>>();
-    case 58: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 3, 1, 1>>();
-    case 59: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 3, 1, 4>>();
-    case 60: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<1, 2, 0, 2>>();
-    case 61: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<3, 2, 1, 2>>();
-    case 62: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<2, 2, 2, 2>>();
-    case 63: return make_pvariant2r<30, 12, Levels<0, 18, 30, 55, 83>>();
-    case 64: return make_pvariant2r<30, 12, Levels<0, 18, 30, 45, 83>>();
-    case 65: return make_pvariant2r<30, 12, Levels<0, 17, 30, 51, 83>>();
-    case 66: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>>();
-    case 67: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<2, 3, 1, 1>>();
-    case 68: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<3, 2, 1, 1>>();
-    case 69: return make_pvariant2r<30, 12, Levels<0, 18, 30, 51, 83>, Tune<3, 2, 1, 1>>();
-    case 70: return make_pvariant2r<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>>();
-    case 71: return make_pvariant2r<30, 12, Levels<0, 19, 30, 53, 83>, Tune<2, 3, 1, 1>>();
-    case 72: return make_pvariant2r<30, 12, Levels<0, 19, 30, 49, 83>, Tune<2, 3, 1, 1>>();
-    case 73: return make_pvariant2r<29, 12, Levels<0, 19, 29, 51, 83>, Tune<2, 3, 1, 1>>();
-    case 74: return make_pvariant2r<30, 12, Levels<0, 19, 30, 51, 83>, Tune<3, 3, 1, 1>>();
-    /* 75..80: queues 1.. in global memory; 42 or 38 primes in LDS */
-    case 75: return make_pvariant2rg<42, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 76: return make_pvariant2rg<42, 12, Levels<0, 20, 30, 42, 60, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 77: return make_pvariant2rg<42, 12, Levels<0, 20, 32, 42, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 78: return make_pvariant2rg<42, 12, Levels<0, 20, 42, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 79: return make_pvariant2rg<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 80: return make_pvariant2rg<38, 12, Levels<0, 20, 30, 38, 55, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 81..85: only the last queue in global memory; 37 primes in LDS */
-    case 81: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 82: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 37, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 83: return make_pvariant2rg<37, 12, Levels<0, 20, 37, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 84: return make_pvariant2rg<37, 12, Levels<0, 20, 30, 44, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 85: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 86: return make_pvariant2rg<37, 12, Levels<0, 20, 34, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 87: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 55, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 88: return make_pvariant2rg<37, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 89: return make_pvariant2rg<37, 12, Levels<0, 19, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 90: return make_pvariant2rg<37, 12, Levels<0, 20, 32, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 91: return make_pvariant2rg<37, 12, Levels<0, 21, 33, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    /* 92..97: 512-byte rotation rows; 16 waves where the registers allow */
-    case 92: return make_pvariant2rw<30, 12, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 93: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 2>, 1>();
-    case 94: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 95: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 96: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, Tune<2, 3, 1, 4>, 1>();
-    case 97: return make_pvariant2rw<27, 12, Levels<0, 20, 27, 51, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 98: return make_pvariant2rw<34, 12, Levels<0, 20, 34, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 99: return make_pvariant2rw<34, 12, Levels<0, 20, 32, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 100: return make_pvariant2rw<34, 12, Levels<0, 20, 34, 55, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 101: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 47, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 102: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 42, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 103, 104: 95 with the last 4 or 2 level-0 primes read through L1 */
-    case 103: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 4>, 1>();
-    case 104: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneL1<2, 3, 1, 1, 2>, 1>();
-    /* 105: 95 with the packed level-0 gathers on live lanes only */
-    case 105: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
-    /* 106, 107: the same at 16 waves per CU (the registers now fit without
-       spills in the position loop), fewer primes in LDS for the queues */
-    case 106: return make_pvariant2rw<30, 16, Levels<0, 20, 30, 51, 83>, TuneSkip<2, 3, 1, 1>, 1>();
-    case 107: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 2>, 1>();
-    case 108: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<2, 3, 1, 4>, 1>();
-    case 109: return make_pvariant2rw<31, 16, Levels<0, 20, 31, 51, 83>, TuneSkip<1, 2, 0, 2>, 1>();
-    /* 110..113: 95 with five levels (one more compaction, its queue in
-       global memory) */
-    case 110: return make_pvariant2rw<34, 12, Levels<0, 20, 26, 33, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 111: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 112: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 42, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 113: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 114..118: six and seven levels */
-    case 114: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 115: return make_pvariant2rw<34, 12, Levels<0, 20, 33, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 116: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 117: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 118: return make_pvariant2rw<34, 12, Levels<0, 20, 30, 34, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 119..123: eight to ten levels */
-    case 119: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 60, 70, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 120: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 40, 46, 51, 60, 70, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 121: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 58, 66, 74, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 122: return make_pvariant2rw<34, 12, Levels<0, 20, 27, 31, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 123: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 40, 46, 53, 61, 71, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 124..127: split points near 117's */
-    case 124: return make_pvariant2rw<34, 12, Levels<0, 20, 29, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 125: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 43, 53, 68, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 126: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 41, 49, 63, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 127: return make_pvariant2rw<34, 12, Levels<0, 21, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 128..131: 117 with a level 0 of 19 primes, or other wave priorities */
-    case 128: return make_pvariant2rw<34, 12, Levels<0, 19, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 129: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 2, 1>, 1>();
-    case 130: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 0, 1>, 1>();
-    case 131: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<1, 3, 1, 1>, 1>();
-    /* 132..134: 117 with the global queues from 3, 2 or 4 on non-temporal */
-    case 132: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 3>, 1>();
-    case 133: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 2>, 1>();
-    case 134: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, TuneNT<2, 3, 1, 1, 4>, 1>();
-    /* 135..137: level 0 on the 16 rotated primes only, the packed 139..157
-       a level of their own (slower: twice the queue-0 traffic) */
-    case 135: return make_pvariant2rw<34, 12, Levels<0, 16, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 136: return make_pvariant2rw<34, 12, Levels<0, 16, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 3, 1>, 1>();
-    case 137: return make_pvariant2rw<34, 12, Levels<0, 16, 22, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    /* 138, 139: seven levels with 27 primes in LDS, the level-1 queue in LDS
-       (138: a third of 117's L2 write-backs, 2.6 % slower) or in global
-       memory (139) */
-    case 138: return make_pvariant2rw<27, 12, Levels<0, 20, 27, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 2>();
-    case 139: return make_pvariant2rw<27, 12, Levels<0, 20, 27, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>();
-    case 28: return make_pvariant2<29, 16, Levels<0, 18, 29, 51, 83>>();
-    default: return make_pvariant2rw<34, 12, Levels<0, 20, 28, 34, 42, 51, 66, 83>, Tune<2, 3, 1, 1>, 1>(); /* = 117 */
-  }
+  bool (*const parts[])(int, PersistVariant &) = {pick_pvariant_1, pick_pvariant_2, pick_pvariant_3,
+                                                   pick_pvariant_4, pick_pvariant_5};
+  PersistVariant v{0, 0, 0, nullptr};
+  for (const int want : {id, 117})
+    for (auto part : parts)
+      if (part(want, v)) return v;
+  return v; /* not reached: 117 is in the table */
 }
 
 }  // namespace

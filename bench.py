@@ -109,7 +109,8 @@ VALU_ISSUE_CEILING = 1.256
 # the deeper compaction levels, written back from L2 and read again.
 TRAFFIC_BYTES_PER_K = 3.4869e10
 # The batched engine's sieve when no variant is named (pick_pvariant(0) in
-# ap26_amd/csrc/ap26_batch.hip); used to name the kernel on the line.
+# ap26_amd/csrc/ap26_batch.hip, the table in ap26_pvariants_*.hip); used to
+# name the kernel on the line.
 DEFAULT_BATCH_VARIANT = 117
 # Algorithmic bytes of the sieve: each survivor written once (8-byte value +
 # 4-byte job index); the per-K tables (10 shifts x 23,783 words x 8 B) and
